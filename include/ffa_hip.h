@@ -1,0 +1,216 @@
+/*
+ * ffa_hip.h -- C ABI of libffa_hip.so, the MI355X (gfx950) implementation of
+ * PyFFA's multiparticle tracking hot path.
+ *
+ * The reference has no compiled boundary: its Python classes call Numba
+ * kernels directly.  Each entry point below replaces one of those calls (or a
+ * fused group of them); the citation on each names the reference interface it
+ * stands in for.  All pointers inside the structs are DEVICE pointers (the
+ * Python side owns the memory as torch tensors and passes data_ptr()), except
+ * where a comment says host.  Every call is asynchronous on the context's
+ * stream and returns 0 on success or a negative FFA_ERR_* code;
+ * ffa_last_error() then describes the failure.  A context is used by one host
+ * thread at a time.
+ */
+#ifndef FFA_HIP_H
+#define FFA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFA_OK 0
+#define FFA_ERR_INVALID (-1)   /* bad argument / shape */
+#define FFA_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define FFA_ERR_STATE (-3)     /* call sequence error (e.g. no machine set) */
+
+#define FFA_MAX_SEG 8
+#define FFA_MAX_POLY 8
+#define FFA_MAX_HARM 8
+#define FFA_MAX_GAP 8
+
+/* Per-particle flag byte (replaces LocalBunch columns 7 and 8 and the
+ * Malloc_SC_survive_flag_Pre/PrePre masks, src/FFAG_Bunch.py:23-62, 241-242). */
+#define FFA_F_INJ 1u
+#define FFA_F_ALIVE 2u
+#define FFA_F_SCPRE 4u
+#define FFA_F_SCPREPRE 8u
+
+/* Particle state as structure-of-arrays (LocalBunch columns 0-5 and 9 plus
+ * the SC endpoint fields Malloc_SC_E*_Pre / _Pre_Pre, src/FFAG_Bunch.py:
+ * 144-149).  fidot holds dphi/dt [rad/s] for the Boris pusher and E_total [J]
+ * for RK4 (src/FFAG_ParasAndConversion.py:202-269).  esc_pre / esc_prepre are
+ * 3*n values, planes (E_r, E_z, E_phi); their element type is double for the
+ * FP64 path and float for the FP32 tier (esc_f32 = 1). */
+typedef struct {
+    double *r, *vr, *z, *vz, *fi, *fidot, *rf_phase;
+    uint8_t *flags;
+    void *esc_pre, *esc_prepre;
+    int64_t n;
+    int32_t esc_f32;
+} ffa_particles;
+
+/* Fields and constants (what FFAG_GlobalParameters carries into the kernels:
+ * FFAG_Bfield_analytical src/FFAG_Field.py:641-706, FFAG_EField_new
+ * src/FFAG_Field.py:168-198, FFAG_BField_Error src/FFAG_Field.py:965-998,
+ * constants src/FFAG_ParasAndConversion.py:13-19). */
+typedef struct {
+    /* Taylor maps, device, interleaved [r_size][fi_size][max_order][3] with the
+     * component order (Bz, Bfi, Br); orders 1..max_order only. */
+    const void *taylor;
+    int32_t taylor_f32;         /* element type of taylor: 0 double, 1 float */
+    int32_t max_order;
+    int32_t r_size, fi_size;
+    double r_min, r_step, fi_min, fi_step;
+    /* analytic order-0 field */
+    double B0, k_value;
+    int32_t nsectors, n_seg;
+    double convex[FFA_MAX_SEG][5];     /* (x0, x1, W0, y01, y02) rows */
+    int32_t n_spoly, n_sppoly;
+    double s_poly[FFA_MAX_POLY], sp_poly[FFA_MAX_POLY];   /* highest degree first */
+    /* harmonic errors */
+    int32_t n_harm;
+    double harm_m[FFA_MAX_HARM], harm_amp[FFA_MAX_HARM], harm_phase[FFA_MAX_HARM];
+    double harm_rmin[FFA_MAX_HARM], harm_rmax[FFA_MAX_HARM];
+    /* fixed RF gaps */
+    int32_t rf_enable, n_gap;
+    double acc_etot;                    /* 1 / gap_width */
+    double gap_theta[FFA_MAX_GAP];      /* gap azimuth [rad] */
+    double gap_px[FFA_MAX_GAP][4], gap_py[FFA_MAX_GAP][4];
+    double gap_box[FFA_MAX_GAP][4];     /* x_min, x_max, y_min, y_max */
+    double rf_shift[FFA_MAX_GAP];
+    /* aperture */
+    int32_t aperture_enable;
+    double aperture[4];                 /* rmin, rmax, zmin, zmax */
+    /* constants */
+    double q, c, m0, E0;
+} ffa_machine;
+
+/* One push step (scalars the reference computes on the host per step). */
+typedef struct {
+    double t;          /* time at the start of the step [s] */
+    double dt;         /* step [s] */
+    double dRF;        /* RF phase advance 2 pi f(t) dt (Boris) */
+    double v0;         /* gap voltage V0(t) [V] */
+    double sc_s;       /* SC extrapolation factor k/SC_step (0 before the 2nd solve) */
+    /* RK4 only: f(t) and V0(t) at t, t+h/2, t+h */
+    double rk_rate[3]; /* 2 pi f */
+    double rk_v0[3];
+} ffa_step;
+
+/* SC solve parameters (FFAG_Bunch attributes read by Build_2_5D_SliceGrid_sbs,
+ * src/FFAG_Bunch.py:1157-1352). */
+typedef struct {
+    int32_t ns, nx, nz;
+    double sigma_ratio;
+    double q_macro;        /* marcosize * q */
+    int32_t rk;            /* gamma from E_total (GetLocalCoordinates_2p5_rk) */
+    int32_t first;         /* first solve of the run (steps_i == 0) */
+    double s_last;         /* extrapolation factor of the step before this solve */
+    int32_t bits;          /* fixed-point fractional bits per CIC axis */
+    /* SEO tables, device, row-major (n_ek, n_fi) */
+    const double *seo_r, *seo_pr;
+    const double *seo_ek_axis, *seo_perimeter;   /* n_ek each */
+    int32_t n_ek, n_fi;
+    double ek0, dek, fi0, dfi;
+    double E0_MeV;         /* rest energy used for Ek = (gamma-1) E0 */
+} ffa_sc_params;
+
+/* Scalars a solve leaves on the device (read back only when asked). */
+typedef struct {
+    double count, sum_gamma, fi_s, fi_e, slice_step, mean_gamma_m1;
+    double sum_x, sum_z, sum_x2, sum_z2;
+    double xmin, xmax, zmin, zmax, dx, dz;
+    double bunch_length, perimeter, qscale;
+    double bad_grid;
+    double arc_min[4], arc_max[4];
+    double pad[4];
+} ffa_sc_result;
+
+/* Device workspace of one SC solve; ffa_sc_workspace_layout() gives the
+ * byte offset of each region inside one caller-allocated block. */
+typedef struct {
+    size_t total;
+    size_t grid;        /* int64 (ns, nz, nx) fixed-point charge */
+    size_t cell_count;  /* int32 (ns, nz, nx) particles per CIC base cell */
+    size_t cbuf;        /* complex64 (ceil(ns/2), nz, nx) packed slice pairs */
+    size_t phi;         /* float (ns, nz, nx) potential */
+    size_t ex, ez;      /* float (ns, nz, nx) fields */
+    size_t px;          /* double n: x = r - r0 */
+    size_t ppr0;        /* double n: pr0 */
+    size_t pslice;      /* int32 n: slice index */
+    size_t slice_count; /* int32 ns */
+    size_t partials;    /* double [FFA_SC_BLOCKS][16] */
+    size_t result;      /* ffa_sc_result */
+} ffa_sc_layout;
+
+#define FFA_SC_BLOCKS 1024
+
+/* Context lifecycle.  stream may be NULL (default stream). */
+int ffa_create(int device, void *stream, void **ctx_out);
+int ffa_destroy(void *ctx);
+const char *ffa_last_error(void *ctx);
+int ffa_set_stream(void *ctx, void *stream);
+
+/* FFAG_Bfield_analytical / FFAG_EField_new / FFAG_BField_Error /
+ * GlobalParameters.AddAperture -> one machine description
+ * (examples/main_track.py:247-270).  Host pointer; copied. */
+int ffa_set_machine(void *ctx, const ffa_machine *m);
+
+/* FFAG_Bunch.InjectParticles, src/FFAG_Bunch.py:265-303: the host finds the
+ * contiguous index range [begin, end) by searchsorted; this sets the
+ * injected and alive flags of that range. */
+int ffa_inject(void *ctx, const ffa_particles *p, int64_t begin, int64_t end);
+
+/* One Boris step in FP64: the SC extrapolation of src/FFAG_track.py:449-478
+ * fused with Boris_solve, src/FFAG_track.py:948-1049 (fast_mod_parallel,
+ * GetBfields_Njit, Bmap_add_all_orders, harmonics, emap_kernel per gap,
+ * cylindrical_to_cartesian, boris_push_cartesian_njit,
+ * convert_cartesian_to_cylindrical). */
+int ffa_push_boris_f64(void *ctx, const ffa_particles *p, const ffa_step *s);
+
+/* One classic RK4 step in FP64: rk4_step_malloc + RK4_Post_Step_njit,
+ * src/FFAG_track.py:51-61, src/FFAG_MathTools.py:2205-2299, with the four
+ * FunctionForAccelerationBunch_dt evaluations (src/FFAG_track.py:1743-1801)
+ * kept in registers. */
+int ffa_push_rk4_f64(void *ctx, const ffa_particles *p, const ffa_step *s);
+
+/* nsteps consecutive calls of ffa_inject + ffa_push_boris_f64 (rk4 == 0) or
+ * ffa_push_rk4_f64 (rk4 != 0) issued from C: the reference loop body of
+ * src/FFAG_track.py:331-556 for steps without an SC solve, one kernel launch
+ * per step, without a host round trip per step.  steps: host array of
+ * nsteps entries; inj: host array of nsteps (begin, end) pairs. */
+int ffa_push_f64_multi(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                       int32_t nsteps, int32_t rk4);
+
+/* nsteps consecutive Boris steps of the FP32-field tier in one launch
+ * (state and RF phase in FP64, field evaluation and SC kick in FP32, state
+ * held in registers between steps).  steps: host array of nsteps entries.
+ * inj: host array of nsteps (begin, end) pairs injected before each step
+ * (begin == end for none).  Same physics as nsteps calls of
+ * ffa_push_boris_f64 up to FP32 field rounding. */
+int ffa_push_boris_fused_f32(void *ctx, const ffa_particles *p, const ffa_step *steps,
+                             const int64_t *inj, int32_t nsteps);
+
+/* Workspace layout of an SC solve for n particles on an (ns, nz, nx) grid. */
+int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *out);
+
+/* FFAG_Bunch.Build_2_5D_SliceGrid_sbs, src/FFAG_Bunch.py:1157-1352, plus
+ * the endpoint roll of src/FFAG_track.py:356-382: survivor coordinates,
+ * covering arc, SEO lookup, slicing and moments, fixed-point CIC deposit,
+ * periodic FFT Poisson solve per slice, -grad(phi)/gamma^2, bilinear gather
+ * with the pr0 projection, then Pre/PrePre endpoint fields and masks rolled
+ * in place.  workspace: device block laid out by ffa_sc_workspace_layout. */
+int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace);
+
+/* Stop-condition reduction of src/FFAG_track.py:559-577: out (device, 2
+ * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
+int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+// C-ABI entry points of libffa_hip.so (declared in include/ffa_hip.h).
+// Argument validation happens here, on the host, before any launch; C++
+// exceptions never cross the boundary.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include "ffa_internal.h"
+
+int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *L);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(ffa_ctx_s *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    g_err = buf;
+    return code;
+}
+
+int hip_check(ffa_ctx_s *c, hipError_t e, const char *what) {
+    if (e == hipSuccess) return FFA_OK;
+    return fail(c, FFA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+int check_particles(ffa_ctx_s *c, const ffa_particles *p) {
+    if (!p) return fail(c, FFA_ERR_INVALID, "particles is NULL");
+    if (p->n < 0) return fail(c, FFA_ERR_INVALID, "negative particle count");
+    if (p->n == 0) return FFA_OK;
+    if (!p->r || !p->vr || !p->z || !p->vz || !p->fi || !p->fidot || !p->rf_phase || !p->flags || !p->esc_pre ||
+        !p->esc_prepre)
+        return fail(c, FFA_ERR_INVALID, "a particle array pointer is NULL");
+    return FFA_OK;
+}
+
+ffa_ctx_s *as_ctx(void *v) { return reinterpret_cast<ffa_ctx_s *>(v); }
+
+}  // namespace
+
+extern "C" {
+
+int ffa_create(int device, void *stream, void **ctx_out) {
+    if (!ctx_out) return fail(nullptr, FFA_ERR_INVALID, "ctx_out is NULL");
+    *ctx_out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return fail(nullptr, FFA_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(nullptr, FFA_ERR_INVALID, "device %d out of range (%d)", device, ndev);
+    ffa_ctx_s *c = new (std::nothrow) ffa_ctx_s();
+    if (!c) return fail(nullptr, FFA_ERR_HIP, "out of host memory");
+    c->device = device;
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->dev), FFA_CTX_DEV_BYTES);
+    if (e != hipSuccess) {
+        int rc = fail(nullptr, FFA_ERR_HIP, "context allocation: %s", hipGetErrorString(e));
+        delete c;
+        return rc;
+    }
+    c->dev_bytes = FFA_CTX_DEV_BYTES;
+    *ctx_out = c;
+    return FFA_OK;
+}
+
+int ffa_destroy(void *ctx) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return FFA_OK;
+    (void)hipSetDevice(c->device);
+    if (c->dev) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(c->dev);
+    }
+    delete c;
+    return FFA_OK;
+}
+
+const char *ffa_last_error(void *ctx) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    return c ? c->err.c_str() : g_err.c_str();
+}
+
+int ffa_set_stream(void *ctx, void *stream) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(nullptr, FFA_ERR_INVALID, "ctx is NULL");
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    return FFA_OK;
+}
+
+int ffa_set_machine(void *ctx, const ffa_machine *m) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !m) return fail(c, FFA_ERR_INVALID, "ctx or machine is NULL");
+    if (m->nsectors <= 0) return fail(c, FFA_ERR_INVALID, "nsectors must be positive");
+    if (m->n_seg < 0 || m->n_seg > FFA_MAX_SEG) return fail(c, FFA_ERR_INVALID, "n_seg out of range");
+    if (m->n_spoly < 0 || m->n_spoly > FFA_MAX_POLY || m->n_sppoly < 0 || m->n_sppoly > FFA_MAX_POLY)
+        return fail(c, FFA_ERR_INVALID, "spiral polynomial degree out of range");
+    if (m->n_harm < 0 || m->n_harm > FFA_MAX_HARM) return fail(c, FFA_ERR_INVALID, "n_harm out of range");
+    if (m->n_gap < 0 || m->n_gap > FFA_MAX_GAP) return fail(c, FFA_ERR_INVALID, "n_gap out of range");
+    if (m->max_order < 0) return fail(c, FFA_ERR_INVALID, "max_order negative");
+    if (m->max_order > 0) {
+        if (!m->taylor) return fail(c, FFA_ERR_INVALID, "max_order > 0 but no Taylor map");
+        if (m->r_size < 2 || m->fi_size < 2) return fail(c, FFA_ERR_INVALID, "Taylor map needs >= 2 nodes per axis");
+        if (!(m->r_step > 0.0) || !(m->fi_step > 0.0)) return fail(c, FFA_ERR_INVALID, "map steps must be positive");
+    }
+    if (m->rf_enable && m->n_gap > 0 && !(m->acc_etot == m->acc_etot))
+        return fail(c, FFA_ERR_INVALID, "acc_etot is NaN");
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dev + FFA_CTX_MACHINE_OFF, m, sizeof(ffa_machine), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "ffa_set_machine");
+    c->host_machine = *m;
+    c->have_machine = true;
+    return FFA_OK;
+}
+
+int ffa_inject(void *ctx, const ffa_particles *p, int64_t begin, int64_t end) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(nullptr, FFA_ERR_INVALID, "ctx is NULL");
+    if (int rc = check_particles(c, p)) return rc;
+    if (begin < 0 || end > p->n || begin > end)
+        return fail(c, FFA_ERR_INVALID, "inject range [%lld, %lld) outside [0, %lld)", (long long)begin,
+                    (long long)end, (long long)p->n);
+    return hip_check(c, launch_inject(c, p, begin, end), "ffa_inject");
+}
+
+int ffa_push_boris_f64(void *ctx, const ffa_particles *p, const ffa_step *s) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !s) return fail(c, FFA_ERR_INVALID, "ctx or step is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP64 push needs a double Taylor map");
+    if (int rc = check_particles(c, p)) return rc;
+    if (p->esc_f32) return fail(c, FFA_ERR_INVALID, "the FP64 push needs double SC endpoints");
+    return hip_check(c, launch_boris_f64(c, p, s), "ffa_push_boris_f64");
+}
+
+int ffa_push_rk4_f64(void *ctx, const ffa_particles *p, const ffa_step *s) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !s) return fail(c, FFA_ERR_INVALID, "ctx or step is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP64 push needs a double Taylor map");
+    if (int rc = check_particles(c, p)) return rc;
+    if (p->esc_f32) return fail(c, FFA_ERR_INVALID, "the FP64 push needs double SC endpoints");
+    return hip_check(c, launch_rk4_f64(c, p, s), "ffa_push_rk4_f64");
+}
+
+int ffa_push_f64_multi(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                       int32_t nsteps, int32_t rk4) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !steps || !inj) return fail(c, FFA_ERR_INVALID, "ctx, steps or inj is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP64 push needs a double Taylor map");
+    if (int rc = check_particles(c, p)) return rc;
+    if (p->esc_f32) return fail(c, FFA_ERR_INVALID, "the FP64 push needs double SC endpoints");
+    if (nsteps < 0) return fail(c, FFA_ERR_INVALID, "nsteps negative");
+    for (int k = 0; k < nsteps; ++k)
+        if (inj[2 * k] < 0 || inj[2 * k + 1] > p->n || inj[2 * k] > inj[2 * k + 1])
+            return fail(c, FFA_ERR_INVALID, "inject range of step %d invalid", k);
+    for (int k = 0; k < nsteps; ++k) {
+        hipError_t e = launch_inject(c, p, inj[2 * k], inj[2 * k + 1]);
+        if (e == hipSuccess) e = rk4 ? launch_rk4_f64(c, p, &steps[k]) : launch_boris_f64(c, p, &steps[k]);
+        if (e != hipSuccess) return hip_check(c, e, "ffa_push_f64_multi");
+    }
+    return FFA_OK;
+}
+
+int ffa_push_boris_fused_f32(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                             int32_t nsteps) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !steps || !inj) return fail(c, FFA_ERR_INVALID, "ctx, steps or inj is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && !c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP32 tier needs a float Taylor map");
+    if (int rc = check_particles(c, p)) return rc;
+    if (nsteps < 0 || nsteps > FFA_MAX_FUSED || (size_t)nsteps * 16 > FFA_CTX_INJ_BYTES)
+        return fail(c, FFA_ERR_INVALID, "nsteps %d outside [0, %d]", nsteps, (int)(FFA_CTX_INJ_BYTES / 16));
+    for (int k = 0; k < nsteps; ++k)
+        if (inj[2 * k] < 0 || inj[2 * k + 1] > p->n || inj[2 * k] > inj[2 * k + 1])
+            return fail(c, FFA_ERR_INVALID, "inject range of step %d invalid", k);
+    if (nsteps == 0) return FFA_OK;
+    hipError_t e = hipMemcpyAsync(c->dev + FFA_CTX_STEPS_OFF, steps, sizeof(ffa_step) * nsteps,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dev + FFA_CTX_INJ_OFF, inj, sizeof(int64_t) * 2 * nsteps, hipMemcpyHostToDevice,
+                           c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "ffa_push_boris_fused_f32 (step table)");
+    return hip_check(c, launch_boris_fused_f32(c, p, nsteps), "ffa_push_boris_fused_f32");
+}
+
+int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *out) {
+    if (!out) return fail(nullptr, FFA_ERR_INVALID, "out is NULL");
+    if (n < 0 || ns < 2) return fail(nullptr, FFA_ERR_INVALID, "need n >= 0 and ns >= 2");
+    if (!pow2_in(nx, 32, 1024) || !pow2_in(nz, 32, 1024))
+        return fail(nullptr, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", nx, nz);
+    return ffa_sc_layout_compute(n, ns, nz, nx, out);
+}
+
+int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !sp || !workspace) return fail(c, FFA_ERR_INVALID, "ctx, params or workspace is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (int rc = check_particles(c, p)) return rc;
+    if (sp->ns < 2) return fail(c, FFA_ERR_INVALID, "ns must be >= 2");
+    if (!pow2_in(sp->nx, 32, 1024) || !pow2_in(sp->nz, 32, 1024))
+        return fail(c, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", sp->nx, sp->nz);
+    if (sp->bits < 1 || sp->bits > 30) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 30]");
+    if ((double)p->n * (double)(1LL << (2 * sp->bits)) >= 9.2e18)
+        return fail(c, FFA_ERR_INVALID, "fixed-point scale 2^%d too large for %lld particles", 2 * sp->bits,
+                    (long long)p->n);
+    if (!sp->seo_r || !sp->seo_pr || !sp->seo_ek_axis || !sp->seo_perimeter || sp->n_ek < 2 || sp->n_fi < 2)
+        return fail(c, FFA_ERR_INVALID, "SEO tables missing or smaller than 2x2");
+    ffa_sc_layout L;
+    ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
+    return hip_check(c, launch_sc_solve(c, p, sp, workspace, &L), "ffa_sc_solve");
+}
+
+int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !out_dev) return fail(c, FFA_ERR_INVALID, "ctx or out is NULL");
+    if (int rc = check_particles(c, p)) return rc;
+    return hip_check(c, launch_reduce_phi(c, p, out_dev), "ffa_reduce_phi");
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Internal declarations shared by the translation units of libffa_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "ffa_hip.h"
+
+#define FFA_PI 3.141592653589793
+#define FFA_TWO_PI (2.0 * FFA_PI)
+
+struct ffa_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool have_machine = false;
+    ffa_machine host_machine{};
+    // small device block owned by the context:
+    //   [0, 8K)        ffa_machine
+    //   [8K, 8K+256K)  step table for fused launches
+    //   [..., +64K)    injection ranges for fused launches
+    //   [..., +64K)    reduction partials / scalars
+    char *dev = nullptr;
+    size_t dev_bytes = 0;
+};
+
+static constexpr size_t FFA_CTX_MACHINE_OFF = 0;
+static constexpr size_t FFA_CTX_STEPS_OFF = 8192;
+static constexpr size_t FFA_CTX_STEPS_BYTES = 256 * 1024;
+static constexpr size_t FFA_CTX_INJ_OFF = FFA_CTX_STEPS_OFF + FFA_CTX_STEPS_BYTES;
+static constexpr size_t FFA_CTX_INJ_BYTES = 64 * 1024;
+static constexpr size_t FFA_CTX_RED_OFF = FFA_CTX_INJ_OFF + FFA_CTX_INJ_BYTES;
+static constexpr size_t FFA_CTX_RED_BYTES = 64 * 1024;
+static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_RED_OFF + FFA_CTX_RED_BYTES;
+static constexpr int FFA_MAX_FUSED = (int)(FFA_CTX_STEPS_BYTES / sizeof(ffa_step));
+
+inline const ffa_machine *ctx_machine_dev(ffa_ctx_s *c) {
+    return reinterpret_cast<const ffa_machine *>(c->dev + FFA_CTX_MACHINE_OFF);
+}
+
+// launchers implemented in the .hip files; return a hipError_t
+hipError_t launch_inject(ffa_ctx_s *c, const ffa_particles *p, int64_t b, int64_t e);
+hipError_t launch_boris_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s);
+hipError_t launch_rk4_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s);
+hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps);
+hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_dev);
+hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp,
+                           void *ws, const ffa_sc_layout *L);

@@ -1,0 +1,368 @@
+// FP64 push kernels of the tracking hot path (gfx950).
+//
+// This translation unit is compiled with -ffp-contract=off: every expression
+// keeps the reference's operation order without fused multiply-adds, so the
+// FP64 path differs from the CPU restatement only by the last-ulp behaviour
+// of the transcendental functions.
+//
+// Layout: one thread per particle, structure-of-arrays FP64 state (each
+// array read and written once per step, fully coalesced), one flag byte per
+// particle, Taylor maps interleaved [r][phi][order][Bz,Bfi,Br] so the two
+// corner pairs of a bilinear gather are two contiguous runs of
+// 2*max_order*3 doubles.
+#include "ffa_internal.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+struct F64Fields {
+    double Bz, Br, Bfi;
+};
+
+// Order-0 analytic field, GetBfields_Njit (src/FFAG_Field.py:1333-1426).
+__device__ __forceinline__ void order0_f64(const ffa_machine &m, double ri, double fii, double zi,
+                                           double &Bz, double &Br, double &Bfi) {
+    const double sector_period = 2.0 * FFA_PI / (double)m.nsectors;
+    double val = 0.0;
+    for (int a = 0; a < m.n_spoly; ++a) val = val * ri + m.s_poly[a];
+    const double spiral = val;
+    val = 0.0;
+    for (int a = 0; a < m.n_sppoly; ++a) val = val * ri + m.sp_poly[a];
+    const double spiral_p = val;
+    double B0r, B0r_p;
+    if (m.k_value == 0.0) {
+        B0r = m.B0;
+        B0r_p = 0.0;
+    } else {
+        B0r = m.B0 * pow(ri, m.k_value);
+        B0r_p = m.B0 * m.k_value * pow(ri, m.k_value - 1.0);
+    }
+    double fs = fii - spiral;
+    fs -= sector_period * floor(fs / sector_period);
+    double F = 0.0, Fp = 0.0;
+    for (int j = 0; j < m.n_seg; ++j) {
+        const double x0 = m.convex[j][0], x1 = m.convex[j][1], W0 = m.convex[j][2];
+        const double yc = m.convex[j][4] * fabs(m.convex[j][3]);
+        const double ks = 16.0 / W0;
+        const double s1 = 1.0 / (1.0 + exp(-ks * (fs - x0)));
+        const double s2 = 1.0 / (1.0 + exp(ks * (fs - x1)));
+        F += yc * s1 * s2;
+        const double s1p = ks * s1 * (1.0 - s1);
+        const double s2p = -ks * s2 * (1.0 - s2);
+        Fp += yc * (s1p * s2 + s1 * s2p);
+    }
+    Bz = B0r * F;
+    Bfi = B0r * Fp / ri * zi;
+    Br = (B0r_p * F - B0r * Fp * spiral_p) * zi;
+}
+
+// Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) on the interleaved map.
+__device__ __forceinline__ void taylor_add_f64(const ffa_machine &m, double x, double y, double z,
+                                               double &oBz, double &oBfi, double &oBr) {
+    const int NX = m.r_size, NY = m.fi_size, no = m.max_order;
+    double fx = (x - m.r_min) / m.r_step;
+    double fy = (y - m.fi_min) / m.fi_step;
+    // int() truncation; non-finite / huge inputs clamp to the map edge
+    fx = fx > -1e9 ? (fx < 1e9 ? fx : 1e9) : -1e9;
+    fy = fy > -1e9 ? (fy < 1e9 ? fy : 1e9) : -1e9;
+    int64_t i = (int64_t)fx, j = (int64_t)fy;
+    if (i < 0) i = 0; else if (i >= NX - 1) i = NX - 2;
+    if (j < 0) j = 0; else if (j >= NY - 1) j = NY - 2;
+    const double tx = (x - (m.r_min + (double)i * m.r_step)) / m.r_step;
+    const double ty = (y - (m.fi_min + (double)j * m.fi_step)) / m.fi_step;
+    const double w00 = (1.0 - tx) * (1.0 - ty), w01 = (1.0 - tx) * ty;
+    const double w10 = tx * (1.0 - ty), w11 = tx * ty;
+    const double *T = reinterpret_cast<const double *>(m.taylor);
+    const int64_t stride_j = 3 * (int64_t)no;
+    const double *a0 = T + (i * NY + j) * stride_j;          // (i, j), (i, j+1) contiguous
+    const double *a1 = T + ((i + 1) * NY + j) * stride_j;    // (i+1, j), (i+1, j+1)
+    const double z2 = z * z;
+    double ze = z2, zo = ze * z;
+    for (int n = 0; n < no; ++n) {
+        const double *p00 = a0 + 3 * n, *p01 = a0 + stride_j + 3 * n;
+        const double *p10 = a1 + 3 * n, *p11 = a1 + stride_j + 3 * n;
+        const double bz = w00 * p00[0] + w01 * p01[0] + w10 * p10[0] + w11 * p11[0];
+        const double bfi = w00 * p00[1] + w01 * p01[1] + w10 * p10[1] + w11 * p11[1];
+        const double br = w00 * p00[2] + w01 * p01[2] + w10 * p10[2] + w11 * p11[2];
+        oBz += bz * ze;
+        oBfi += bfi * zo;
+        oBr += br * zo;
+        if (n != no - 1) { ze *= z2; zo *= z2; }
+    }
+}
+
+// bharmonics_kernel_1d (src/FFAG_Field.py:1541-1587): raw azimuth.
+__device__ __forceinline__ void harmonics_f64(const ffa_machine &m, double r, double phi, double &Bz) {
+    if (r <= 0.0) return;
+    double bz = 0.0;
+    for (int k = 0; k < m.n_harm; ++k)
+        if (m.harm_rmin[k] <= r && r <= m.harm_rmax[k]) bz += m.harm_amp[k] * cos(m.harm_m[k] * phi + m.harm_phase[k]);
+    Bz += bz;
+}
+
+// emap_kernel over the gaps in order (src/FFAG_Utils.py:984-1036 called per
+// gap from src/FFAG_Field.py:296-306): a later gap zeroes the field of a
+// particle outside it, so only the last gap can act.
+__device__ __forceinline__ void emap_f64(const ffa_machine &m, double x, double y, double fi, double v0,
+                                         double &Er, double &Efi, double &Ez, int &gap) {
+    const double acc = m.acc_etot * v0;
+    for (int g = 0; g < m.n_gap; ++g) {
+        if (x < m.gap_box[g][0] || x > m.gap_box[g][1] || y < m.gap_box[g][2] || y > m.gap_box[g][3]) {
+            Er = 0.0; Efi = 0.0; Ez = 0.0;
+            continue;
+        }
+        bool inside = true;
+        double prev = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            const double x0 = m.gap_px[g][j], y0 = m.gap_py[g][j];
+            const double x1 = m.gap_px[g][(j + 1) & 3], y1 = m.gap_py[g][(j + 1) & 3];
+            const double cross = (x1 - x0) * (y - y0) - (y1 - y0) * (x - x0);
+            if (j == 0) prev = cross;
+            else if (cross * prev < 0.0) { inside = false; break; }
+        }
+        if (inside) {
+            const double dth = (m.gap_theta[g] + FFA_PI / 2.0) - fi;
+            Er = acc * cos(dth);
+            Efi = acc * sin(dth);
+            Ez = 0.0;
+            gap = g;
+        } else {
+            Er = 0.0; Efi = 0.0; Ez = 0.0;
+            gap = 0;
+        }
+    }
+}
+
+// Field evaluation of one particle (fast_mod + order 0 + Taylor + harmonics
+// + gap field), for injected-and-alive particles.
+__device__ __forceinline__ void fields_f64(const ffa_machine &m, double r, double fi, double z, double sf,
+                                           double cf, double v0, double &Bz, double &Br, double &Bfi,
+                                           double &Er, double &Efi, double &Ez, int &gap) {
+    const double P = FFA_PI * 2.0 / (double)m.nsectors;
+    const double fim = fi - P * floor(fi / P);
+    order0_f64(m, r, fim, z, Bz, Br, Bfi);
+    if (m.max_order >= 1) taylor_add_f64(m, r, fim, z, Bz, Bfi, Br);
+    if (m.n_harm > 0) harmonics_f64(m, r, fi, Bz);
+    Er = 0.0; Efi = 0.0; Ez = 0.0;
+    if (m.rf_enable) emap_f64(m, r * cf, r * sf, fi, v0, Er, Efi, Ez, gap);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_inject(uint8_t *flags, int64_t b, int64_t e) {
+    const int64_t i = b + blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i < e) flags[i] = (uint8_t)(flags[i] | FFA_F_INJ | FFA_F_ALIVE);
+}
+
+// One FP64 Boris step: A2 extrapolation + A6..A15.
+__global__ __launch_bounds__(BLOCK) void k_boris_f64(ffa_particles p, const ffa_machine *__restrict__ mp,
+                                                    ffa_step s) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const ffa_machine &m = *mp;
+    const int64_t n = p.n;
+    const double r = p.r[i], vr = p.vr[i], z = p.z[i], vz = p.vz[i], fi = p.fi[i], fidot = p.fidot[i];
+    const double rf = p.rf_phase[i];
+    uint8_t fl = p.flags[i];
+    const double *ep = reinterpret_cast<const double *>(p.esc_pre);
+    const double *epp = reinterpret_cast<const double *>(p.esc_prepre);
+    double esr = ep[i], esz = ep[n + i], esf = ep[2 * n + i];
+    if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
+        esr = esr + s.sc_s * (esr - epp[i]);
+        esz = esz + s.sc_s * (esz - epp[n + i]);
+        esf = esf + s.sc_s * (esf - epp[2 * n + i]);
+    }
+    const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
+    const double sinFi = sin(fi), cosFi = cos(fi);
+    double Bz = 0.0, Br = 0.0, Bfi = 0.0, Er = 0.0, Efi = 0.0, Ez = 0.0;
+    int gap = 0;
+    if (live) fields_f64(m, r, fi, z, sinFi, cosFi, s.v0, Bz, Br, Bfi, Er, Efi, Ez, gap);
+    // cylindrical_to_cartesian (src/FFAG_MathTools.py:1969-2014)
+    const double sin_RF = m.rf_enable ? sin(rf + m.rf_shift[gap]) : 0.0;
+    double ezt = Ez * sin_RF, ert = Er * sin_RF, eft = Efi * sin_RF;
+    ezt += esz; ert += esr; eft += esf;
+    const double x = r * cosFi, y = r * sinFi;
+    const double vx = vr * cosFi - r * fidot * sinFi;
+    const double vy = vr * sinFi + r * fidot * cosFi;
+    double xn = x, yn = y, zn = z, vxn = vx, vyn = vy, vzn = vz;
+    if (live) {
+        // boris_push_cartesian_njit (src/FFAG_MathTools.py:1877-1966)
+        if (m.aperture_enable &&
+            (r > m.aperture[1] || r < m.aperture[0] || z > m.aperture[3] || z < m.aperture[2]))
+            fl = (uint8_t)(fl & ~FFA_F_ALIVE);
+        const double Ex = ert * cosFi - eft * sinFi, Ey = ert * sinFi + eft * cosFi, Ezc = ezt;
+        const double Bx = -(Br * cosFi - Bfi * sinFi), By = -(Br * sinFi + Bfi * cosFi), Bzc = -Bz;
+        const double q = m.q, mm = m.m0, c = m.c, dt = s.dt;
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        const double gamma = 1.0 / sqrt(1.0 - v2 / (c * c));
+        double px = gamma * mm * vx, py = gamma * mm * vy, pz = gamma * mm * vz;
+        px += 0.5 * dt * q * Ex; py += 0.5 * dt * q * Ey; pz += 0.5 * dt * q * Ezc;
+        const double p2 = px * px + py * py + pz * pz;
+        const double gm = sqrt(1.0 + p2 / (mm * mm * c * c));
+        const double cb = 0.5 * dt * q / (gm * mm);
+        const double tx = cb * Bx, ty = cb * By, tz = cb * Bzc;
+        const double pxp = px + (py * tz - pz * ty);
+        const double pyp = py + (pz * tx - px * tz);
+        const double pzp = pz + (px * ty - py * tx);
+        const double t2 = tx * tx + ty * ty + tz * tz;
+        const double sc = 2.0 / (1.0 + t2);
+        const double sx = sc * tx, sy = sc * ty, sz = sc * tz;
+        px += (pyp * sz - pzp * sy);
+        py += (pzp * sx - pxp * sz);
+        pz += (pxp * sy - pyp * sx);
+        px += 0.5 * dt * q * Ex; py += 0.5 * dt * q * Ey; pz += 0.5 * dt * q * Ezc;
+        const double p2n = px * px + py * py + pz * pz;
+        const double gn = sqrt(1.0 + p2n / (mm * mm * c * c));
+        vxn = px / (gn * mm); vyn = py / (gn * mm); vzn = pz / (gn * mm);
+        xn = x + vxn * dt; yn = y + vyn * dt; zn = z + vzn * dt;
+    }
+    // convert_cartesian_to_cylindrical (src/FFAG_MathTools.py:2017-2060)
+    const double r_new = sqrt(xn * xn + yn * yn);
+    const double fb = atan2(y, x), fa = atan2(yn, xn);
+    double d = fa - fb;
+    if (d > FFA_PI) d -= 2.0 * FFA_PI;
+    else if (d < -FFA_PI) d += 2.0 * FFA_PI;
+    const double fi_new = fi + d;
+    double sn, cn;
+    sincos(fi_new, &sn, &cn);
+    const double vr_n = vxn * cn + vyn * sn;
+    const double vfi = -vxn * sn + vyn * cn;
+    p.r[i] = r_new;
+    p.vr[i] = vr_n;
+    p.z[i] = zn;
+    p.vz[i] = vzn;
+    p.fi[i] = fi_new;
+    p.fidot[i] = r_new > 1e-12 ? vfi / r_new : 0.0;
+    p.rf_phase[i] = rf + s.dRF;
+    if (fl != p.flags[i]) p.flags[i] = fl;
+}
+
+// RK4_equations_of_motion_njit (src/FFAG_MathTools.py:2205-2266) for one live
+// particle state Y = (r, vr, z, vz, fi, Et, rf).
+__device__ __forceinline__ void rk4_rhs_f64(const ffa_machine &m, const double *Y, double esr, double esz,
+                                            double esf, double rate, double v0, double *d) {
+    const double r = Y[0], rdot = Y[1], z = Y[2], zdot = Y[3], fi = Y[4], Et = Y[5], rf = Y[6];
+    double sf, cf;
+    sincos(fi, &sf, &cf);
+    double Bz = 0.0, Br = 0.0, Bfi = 0.0, Er = 0.0, Efi = 0.0, Ez = 0.0;
+    int gap = 0;
+    fields_f64(m, r, fi, z, sf, cf, v0, Bz, Br, Bfi, Er, Efi, Ez, gap);
+    const double q = m.q, c = m.c;
+    const double ig = m.E0 / Et;
+    const double inv_gamma2 = ig * ig;
+    const double beta2 = 1.0 - inv_gamma2;
+    const double v2 = beta2 * (c * c);
+    const double vperp2 = v2 - rdot * rdot - zdot * zdot;
+    const double fidot = sqrt(vperp2) / r;
+    const double sin_RF = m.rf_enable ? sin(rf + m.rf_shift[gap]) : 0.0;
+    double ez = Ez * sin_RF, er = Er * sin_RF, efi = Efi * sin_RF;
+    ez += esz; er += esr; efi += esf;
+    const double Edv = rdot * er + (r * fidot) * efi + zdot * ez;
+    d[0] = rdot;
+    d[1] = (r * fidot * fidot + q * (c * c) / Et * (er - r * fidot * Bz + zdot * Bfi) - q * rdot / Et * Edv);
+    d[2] = zdot;
+    d[3] = (q * (c * c) / Et * (ez + r * fidot * Br - rdot * Bfi) - q * zdot / Et * Edv);
+    d[4] = fidot;
+    d[5] = q * Edv;
+    d[6] = rate;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rk4_f64(ffa_particles p, const ffa_machine *__restrict__ mp,
+                                                  ffa_step s) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const ffa_machine &m = *mp;
+    const int64_t n = p.n;
+    const uint8_t fl = p.flags[i];
+    const double h = s.dt;
+    const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
+    double Y[7] = {p.r[i], p.vr[i], p.z[i], p.vz[i], p.fi[i], p.fidot[i], p.rf_phase[i]};
+    if (!live) {
+        // RK4_Post_Step_njit: only the RF phase advances
+        const double k1 = h * s.rk_rate[0], k2 = h * s.rk_rate[1], k3 = h * s.rk_rate[1], k4 = h * s.rk_rate[2];
+        p.rf_phase[i] = Y[6] + (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0;
+        return;
+    }
+    const double *ep = reinterpret_cast<const double *>(p.esc_pre);
+    const double *epp = reinterpret_cast<const double *>(p.esc_prepre);
+    double esr = ep[i], esz = ep[n + i], esf = ep[2 * n + i];
+    if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
+        esr = esr + s.sc_s * (esr - epp[i]);
+        esz = esz + s.sc_s * (esz - epp[n + i]);
+        esf = esf + s.sc_s * (esf - epp[2 * n + i]);
+    }
+    double d[7], k1[7], k2[7], k3[7], k4[7], S[7];
+    rk4_rhs_f64(m, Y, esr, esz, esf, s.rk_rate[0], s.rk_v0[0], d);
+    for (int c = 0; c < 7; ++c) { k1[c] = h * d[c]; S[c] = Y[c] + 0.5 * k1[c]; }
+    rk4_rhs_f64(m, S, esr, esz, esf, s.rk_rate[1], s.rk_v0[1], d);
+    for (int c = 0; c < 7; ++c) { k2[c] = h * d[c]; S[c] = Y[c] + 0.5 * k2[c]; }
+    rk4_rhs_f64(m, S, esr, esz, esf, s.rk_rate[1], s.rk_v0[1], d);
+    for (int c = 0; c < 7; ++c) { k3[c] = h * d[c]; S[c] = Y[c] + k3[c]; }
+    rk4_rhs_f64(m, S, esr, esz, esf, s.rk_rate[2], s.rk_v0[2], d);
+    for (int c = 0; c < 7; ++c) { k4[c] = h * d[c]; Y[c] = Y[c] + (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]) / 6.0; }
+    p.r[i] = Y[0]; p.vr[i] = Y[1]; p.z[i] = Y[2]; p.vz[i] = Y[3];
+    p.fi[i] = Y[4]; p.fidot[i] = Y[5]; p.rf_phase[i] = Y[6];
+}
+
+// Stop-condition reduction: fixed block partition, fixed-order combine.
+constexpr int RED_BLOCKS = 512;
+
+__global__ __launch_bounds__(BLOCK) void k_phi_partials(const double *__restrict__ fi, int64_t n,
+                                                       double *__restrict__ part) {
+    __shared__ double ss[BLOCK], sc[BLOCK];
+    double acc = 0.0, cnt = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * BLOCK) {
+        const double v = fi[i];
+        if (v == v) { acc += v; cnt += 1.0; }
+    }
+    ss[threadIdx.x] = acc;
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { ss[threadIdx.x] += ss[threadIdx.x + w]; sc[threadIdx.x] += sc[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = ss[0]; part[2 * blockIdx.x + 1] = sc[0]; }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_phi_final(const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double ss[BLOCK], sc[BLOCK];
+    double a = 0.0, c = 0.0;
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += BLOCK) { a += part[2 * b]; c += part[2 * b + 1]; }
+    ss[threadIdx.x] = a;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { ss[threadIdx.x] += ss[threadIdx.x + w]; sc[threadIdx.x] += sc[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = ss[0]; out[1] = sc[0]; }
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+hipError_t launch_inject(ffa_ctx_s *c, const ffa_particles *p, int64_t b, int64_t e) {
+    if (e <= b) return hipSuccess;
+    hipLaunchKernelGGL(k_inject, dim3(grid_for(e - b)), dim3(BLOCK), 0, c->stream, p->flags, b, e);
+    return hipGetLastError();
+}
+
+hipError_t launch_boris_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s) {
+    if (p->n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_boris_f64, dim3(grid_for(p->n)), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c), *s);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk4_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s) {
+    if (p->n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rk4_f64, dim3(grid_for(p->n)), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c), *s);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_dev) {
+    double *part = reinterpret_cast<double *>(c->dev + FFA_CTX_RED_OFF);
+    hipLaunchKernelGGL(k_phi_partials, dim3(RED_BLOCKS), dim3(BLOCK), 0, c->stream, p->fi, p->n, part);
+    hipLaunchKernelGGL(k_phi_final, dim3(1), dim3(BLOCK), 0, c->stream, part, out_dev);
+    return hipGetLastError();
+}

@@ -1,0 +1,559 @@
+// 2.5D space-charge solve of the tracking hot path (gfx950).
+//
+// Pipeline of one solve (every kernel on the context stream, no host sync):
+//   k_sc_fill          zero the int64 grid, per-cell and per-slice counters
+//   k_sc_stage1        per survivor: gamma, phi mod 2pi; per-block partial
+//                      sums of gamma and the four covering-arc min/max
+//   k_sc_final1        1 block: covering arc, slice step, mean gamma,
+//                      perimeter, bunch length (device-side scalars)
+//   k_sc_stage2        per survivor: SEO r0/pr0, x = r - r0, slice index,
+//                      per-block partial moments, slice histogram
+//   k_sc_final2        1 block: grid box, dx, dz, charge scale
+//   k_sc_deposit       fixed-point CIC deposit with int64 atomics (integer
+//                      sums: bit-identical for any particle order)
+//   k_fft_rows_fwd     int64 -> float, two slices packed as re/im, row FFTs
+//   k_fft_cols_green   column FFT, x periodic Green's function, inverse
+//   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi
+//   k_sc_field         -grad(phi)/gamma^2 with the reference's edge stencils
+//   k_sc_gather_roll   bilinear gather + pr0 projection, endpoint roll
+// Particle-side arithmetic is FP64 in the reference's operation order (this
+// unit is compiled with -ffp-contract=off); the grid is float32 like the
+// reference's (complex64 FFTs).  Reductions use a fixed block partition and
+// a fixed combine order, so a solve is bit-reproducible run to run.
+#include "ffa_internal.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NB = FFA_SC_BLOCKS;
+constexpr double EPS0 = 8.854187817e-12;
+
+__device__ __forceinline__ double gamma_of(const ffa_particles &p, int64_t i, int rk, double c, double E0J) {
+    if (rk) return p.fidot[i] / E0J;
+    const double r = p.r[i], rd = r * p.fidot[i], vr = p.vr[i], vz = p.vz[i];
+    const double v2 = vr * vr + vz * vz + rd * rd;
+    return 1.0 / sqrt(1.0 - v2 / (c * c));
+}
+
+__device__ __forceinline__ double mod2pi(double fi) { return fi - FFA_TWO_PI * floor(fi / FFA_TWO_PI); }
+
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (*sh)[BLOCK], double *v, const int *op /*0 sum,1 min,2 max*/) {
+    for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < NV; ++k) {
+                const double a = sh[k][threadIdx.x], b = sh[k][threadIdx.x + w];
+                sh[k][threadIdx.x] = op[k] == 0 ? a + b : (op[k] == 1 ? (b < a ? b : a) : (b > a ? b : a));
+            }
+        __syncthreads();
+    }
+    for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sc_fill(int64_t *grid, int32_t *cell, size_t ncell, int32_t *slc,
+                                                  int ns) {
+    const size_t stride = (size_t)gridDim.x * BLOCK;
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < ncell; i += stride) {
+        grid[i] = 0;
+        cell[i] = 0;
+    }
+    if (blockIdx.x == 0)
+        for (int s = threadIdx.x; s < ns; s += BLOCK) slc[s] = 0;
+}
+
+// GetLocalCoordinates_2p5(_rk) + divide_arc2's scan (src/FFAG_SC_new.py:74-148, 534-566).
+__global__ __launch_bounds__(BLOCK) void k_sc_stage1(ffa_particles p, int rk, double c, double E0J,
+                                                    double *part) {
+    __shared__ double sh[10][BLOCK];
+    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
+    double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
+        if (!(p.flags[i] & FFA_F_ALIVE)) continue;
+        v[0] += gamma_of(p, i, rk, c, E0J);
+        v[1] += 1.0;
+        const double fm = mod2pi(p.fi[i]);
+        for (int k = 0; k < 4; ++k) {
+            const double a = fm + shifts[k], b = 2.0 * FFA_PI;
+            const double q = a - b * floor(a / b);
+            v[2 + k] = q < v[2 + k] ? q : v[2 + k];
+            v[6 + k] = q > v[6 + k] ? q : v[6 + k];
+        }
+    }
+    const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+    block_reduce<10>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 10; ++k) part[16 * blockIdx.x + k] = v[k];
+}
+
+__device__ double py_mod(double a, double b) {  // Python float % (sign of divisor)
+    double r = fmod(a, b);
+    if (r != 0.0 && ((r < 0.0) != (b < 0.0))) r += b;
+    return r;
+}
+
+__device__ double np_interp(double x, const double *xp, const double *fp, int n) {
+    if (n == 1) return fp[0];
+    if (x < xp[0]) return fp[0];
+    if (x > xp[n - 1]) return fp[n - 1];
+    if (x == xp[n - 1]) return fp[n - 1];
+    int lo = 0, hi = n - 1;            // largest j with xp[j] <= x
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= x) lo = mid; else hi = mid;
+    }
+    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    double r = slope * (x - xp[lo]) + fp[lo];
+    if (r != r) r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+    return r;
+}
+
+// Covering arc (divide_arc2 + merge_multiple_arcs for one rank,
+// src/FFAG_SC_new.py:534-606; src/FFAG_Bunch.py:1203-1215), mean gamma and
+// bunch length (src/FFAG_Bunch.py:1217-1294).
+__global__ __launch_bounds__(BLOCK) void k_sc_final1(const double *part, ffa_sc_params sp, ffa_sc_result *res) {
+    __shared__ double sh[10][BLOCK];
+    double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
+    const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+    for (int b = threadIdx.x; b < NB; b += BLOCK) {
+        const double *q = part + 16 * b;
+        v[0] += q[0];
+        v[1] += q[1];
+        for (int k = 2; k < 6; ++k) v[k] = q[k] < v[k] ? q[k] : v[k];
+        for (int k = 6; k < 10; ++k) v[k] = q[k] > v[k] ? q[k] : v[k];
+    }
+    block_reduce<10>(sh, v, op);
+    if (threadIdx.x != 0) return;
+    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
+    int best = 0;
+    double bs = v[6] - v[2];
+    for (int k = 1; k < 4; ++k)
+        if (v[6 + k] - v[2 + k] < bs) { bs = v[6 + k] - v[2 + k]; best = k; }
+    double s = v[2 + best] - shifts[best], e = v[6 + best] - shifts[best];
+    if (e < s) e += 2.0 * FFA_PI;
+    double fs = py_mod(s, 2.0 * FFA_PI), fe = py_mod(e, 2.0 * FFA_PI);
+    if (fe <= fs) fe += 2.0 * FFA_PI;
+    const double cnt = v[1];
+    const double mgm1 = cnt > 0.0 ? v[0] / cnt - 1.0 : 0.0;
+    const double perim = np_interp(mgm1 * sp.E0_MeV, sp.seo_ek_axis, sp.seo_perimeter, sp.n_ek);
+    res->count = cnt;
+    res->sum_gamma = v[0];
+    for (int k = 0; k < 4; ++k) { res->arc_min[k] = v[2 + k]; res->arc_max[k] = v[6 + k]; }
+    res->fi_s = fs;
+    res->fi_e = fe;
+    res->slice_step = (fe - fs) / (double)(sp.ns - 1);
+    res->mean_gamma_m1 = mgm1;
+    res->perimeter = perim;
+    res->bunch_length = perim * (fe - fs) / FFA_PI / 2.0;
+}
+
+// Bilinear_interp_2D_vect_uniform (src/FFAG_MathTools.py:827-936).
+__device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, double X0, double dX, double Y0,
+                                               double dY, double x, double y) {
+    double fx = (x - X0) / dX, fy = (y - Y0) / dY;
+    fx = fx > -1e9 ? (fx < 1e9 ? fx : 1e9) : -1e9;
+    fy = fy > -1e9 ? (fy < 1e9 ? fy : 1e9) : -1e9;
+    int64_t i = (int64_t)floor(fx), j = (int64_t)floor(fy);
+    if (i < 0) i = 0; else if (i >= NX - 1) i = NX - 2;
+    if (j < 0) j = 0; else if (j >= NY - 1) j = NY - 2;
+    const double xx0 = X0 + (double)i * dX, yy0 = Y0 + (double)j * dY;
+    const double tx = (x - xx0) / dX, ty = (y - yy0) / dY;
+    const double z00 = Z[i * NY + j], z01 = Z[i * NY + j + 1], z10 = Z[(i + 1) * NY + j], z11 = Z[(i + 1) * NY + j + 1];
+    return (1 - tx) * (1 - ty) * z00 + (1 - tx) * ty * z01 + tx * (1 - ty) * z10 + tx * ty * z11;
+}
+
+// SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
+__global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_params sp, double c, double E0J,
+                                                    const ffa_sc_result *res, double *px, double *ppr0,
+                                                    int32_t *pslice, int32_t *slc, double *part) {
+    __shared__ double sh[4][BLOCK];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const double fs = res->fi_s, step = res->slice_step;
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
+        if (!(p.flags[i] & FFA_F_ALIVE)) { pslice[i] = -1; continue; }
+        const double g = gamma_of(p, i, sp.rk, c, E0J);
+        const double fm = mod2pi(p.fi[i]);
+        const double ek = (g - 1) * sp.E0_MeV;
+        const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+        const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+        const double x = p.r[i] - r0, z = p.z[i];
+        v[0] += x; v[1] += z; v[2] += x * x; v[3] += z * z;
+        double f = fm;
+        if (f < fs) f += FFA_TWO_PI;
+        double fsl = (f - fs) / step;
+        fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
+        int64_t s = (int64_t)fsl;
+        if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
+        px[i] = x;
+        ppr0[i] = pr0;
+        pslice[i] = (int32_t)s;
+        atomicAdd(&slc[s], 1);
+    }
+    const int op[4] = {0, 0, 0, 0};
+    block_reduce<4>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[16 * blockIdx.x + 10 + k] = v[k];
+}
+
+// Grid box from the moments (src/FFAG_Bunch.py:1259-1322).
+__global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res) {
+    __shared__ double sh[4][BLOCK];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < NB; b += BLOCK)
+        for (int k = 0; k < 4; ++k) v[k] += part[16 * b + 10 + k];
+    const int op[4] = {0, 0, 0, 0};
+    block_reduce<4>(sh, v, op);
+    if (threadIdx.x != 0) return;
+    const double cnt = res->count;
+    double mx, mz, sx, sz;
+    if (cnt > 1.0) {
+        mx = v[0] / cnt;
+        mz = v[1] / cnt;
+        const double vx = v[2] / cnt - mx * mx, vz = v[3] / cnt - mz * mz;
+        sx = sqrt(vx > 0.0 ? vx : 0.0);
+        sz = sqrt(vz > 0.0 ? vz : 0.0);
+    } else {
+        mx = mz = sx = sz = 0.0;
+    }
+    res->sum_x = v[0]; res->sum_z = v[1]; res->sum_x2 = v[2]; res->sum_z2 = v[3];
+    const double xmin = mx - sp.sigma_ratio * sx, xmax = mx + sp.sigma_ratio * sx;
+    const double zmin = mz - sp.sigma_ratio * sz, zmax = mz + sp.sigma_ratio * sz;
+    const double dx = (xmax - xmin) / sp.nx, dz = (zmax - zmin) / sp.nz;
+    const double bl = res->bunch_length;
+    res->xmin = xmin; res->xmax = xmax; res->zmin = zmin; res->zmax = zmax;
+    res->dx = dx; res->dz = dz;
+    const bool bad = !(dx > 0.0) || !(dz > 0.0) || !(bl > 0.0) || cnt <= 0.0;
+    res->bad_grid = bad ? 1.0 : 0.0;
+    res->qscale = bad ? 0.0 : (sp.q_macro / bl * sp.ns) * (1.0 / (dx * dz));
+}
+
+// Fixed-point CIC deposit (DistributeChargeSBS_Njit cell/weight rule,
+// src/FFAG_SC_new.py:244-312): a = rint(fx 2^bits), b = rint(fz 2^bits);
+// corners get (2^bits-a)(2^bits-b), a(2^bits-b), (2^bits-a)b, ab.
+__global__ __launch_bounds__(BLOCK) void k_sc_deposit(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
+                                                     const double *px, const int32_t *pslice,
+                                                     unsigned long long *grid, int32_t *cell) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const int s = pslice[i];
+    if (s < 0 || res->bad_grid != 0.0) return;
+    const int nx = sp.nx, nz = sp.nz;
+    const double inv_dx = 1.0 / res->dx, inv_dz = 1.0 / res->dz;
+    const double rx = (px[i] - res->xmin) * inv_dx, rz = (p.z[i] - res->zmin) * inv_dz;
+    const double crx = rx > -1e9 ? (rx < 1e9 ? rx : 1e9) : -1e9;
+    const double crz = rz > -1e9 ? (rz < 1e9 ? rz : 1e9) : -1e9;
+    int64_t ix = (int64_t)floor(crx), iz = (int64_t)floor(crz);
+    double fx = rx - (double)ix, fz = rz - (double)iz;
+    if (ix < 0) { ix = 0; fx = 0.0; } else if (ix >= nx - 1) { ix = nx - 2; fx = 1.0; }
+    if (iz < 0) { iz = 0; fz = 0.0; } else if (iz >= nz - 1) { iz = nz - 2; fz = 1.0; }
+    const int64_t one = (int64_t)1 << sp.bits;
+    const double scale = (double)one;
+    const int64_t a = (int64_t)rint(fx * scale), b = (int64_t)rint(fz * scale);
+    const size_t base = (size_t)s * nz * nx + (size_t)iz * nx + (size_t)ix;
+    atomicAdd(&grid[base], (unsigned long long)((one - a) * (one - b)));
+    atomicAdd(&grid[base + 1], (unsigned long long)(a * (one - b)));
+    atomicAdd(&grid[base + nx], (unsigned long long)((one - a) * b));
+    atomicAdd(&grid[base + nx + 1], (unsigned long long)(a * b));
+    atomicAdd(&cell[base], 1);
+}
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT (radix 2, autosort).  A block transforms `lines` lines of
+// length N (N a power of two, N * lines == 1024) held in LDS.
+// ---------------------------------------------------------------------------
+constexpr int FFT_PTS = 1024;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ void make_twiddles(float2 *tw, int N, bool inv) {
+    for (int k = threadIdx.x; k < N / 2; k += BLOCK) {
+        double s, c;
+        sincospi(-2.0 * (double)k / (double)N, &s, &c);
+        tw[k] = make_float2((float)c, (float)(inv ? -s : s));
+    }
+}
+
+// returns the buffer holding the result
+__device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, int N, int logN) {
+    const int half = N >> 1;
+    const int nbf = (FFT_PTS >> 1);   // butterflies per pass over all lines
+    for (int sl = 0; sl < logN; ++sl) {
+        const int s = 1 << sl, m = N >> (sl + 1);
+        for (int t = threadIdx.x; t < nbf; t += BLOCK) {
+            const int line = t / half, b = t - line * half;
+            const int pp = b >> sl, q = b & (s - 1);
+            float2 *xl = x + line * N, *yl = y + line * N;
+            const float2 a = xl[q + s * pp], bb = xl[q + s * (pp + m)];
+            const float2 w = tw[pp * s];
+            yl[q + s * (2 * pp)] = make_float2(a.x + bb.x, a.y + bb.y);
+            yl[q + s * (2 * pp + 1)] = cmul(make_float2(a.x - bb.x, a.y - bb.y), w);
+        }
+        __syncthreads();
+        float2 *tmp = x; x = y; y = tmp;
+    }
+    return x;
+}
+
+// rows: forward FFT along x of the packed slice pair; grid -> cbuf.
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(const int64_t *grid, float2 *cbuf, ffa_sc_params sp,
+                                                       const ffa_sc_result *res, int lognx) {
+    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2];
+    const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
+    const int lines = FFT_PTS / nx;
+    const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
+    const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
+    make_twiddles(tw, nx, false);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+        const int l = e / nx, i = e - l * nx;
+        const int64_t row = row0 + l;
+        const int64_t plane = row / nz, j = row - plane * nz;
+        const int64_t sa = 2 * plane, sb = sa + 1;
+        const float re = (float)((double)grid[(sa * nz + j) * nx + i] * us);
+        const float im = sb < ns ? (float)((double)grid[(sb * nz + j) * nx + i] * us) : 0.0f;
+        b0[e] = make_float2(re, im);
+    }
+    __syncthreads();
+    const float2 *o = lds_fft(b0, b1, tw, nx, lognx);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) cbuf[row0 * nx + e] = o[e];
+}
+
+// columns: forward FFT along z, multiply by the periodic Green's function
+// (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760), inverse FFT.
+__global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const ffa_sc_result *res,
+                                                         int lognz) {
+    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2], twi[FFT_PTS / 2];
+    const int nx = sp.nx, nz = sp.nz;
+    const int W = FFT_PTS / nz;                       // columns per block
+    const int cpp = nx / W;                           // column groups per plane
+    const int64_t plane = blockIdx.x / cpp;
+    const int c0 = (blockIdx.x - plane * cpp) * W;
+    float2 *base = cbuf + plane * (int64_t)nz * nx;
+    make_twiddles(tw, nz, false);
+    make_twiddles(twi, nz, true);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+        const int j = e / W, c = e - j * W;
+        b0[c * nz + j] = base[(int64_t)j * nx + c0 + c];
+    }
+    __syncthreads();
+    float2 *o = lds_fft(b0, b1, tw, nz, lognz);
+    float2 *other = (o == b0) ? b1 : b0;
+    // Green's function in the reference's float32 arithmetic
+    const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
+    const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+        const int c = e / nz, j = e - c * nz;
+        const int ci = c0 + c;
+        const int idx = ci <= nx / 2 ? ci : ci - nx;
+        const int idz = j <= nz / 2 ? j : j - nz;
+        const float kx = (float)((double)dkx * idx), kz = (float)((double)dkz * idz);
+        const float kz2 = kz * kz;
+        const float k2 = kx * kx + kz2;
+        const float g = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
+        o[e] = make_float2(o[e].x * g, o[e].y * g);
+    }
+    __syncthreads();
+    float2 *r = lds_fft(o, other, twi, nz, lognz);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+        const int j = e / W, c = e - j * W;
+        base[(int64_t)j * nx + c0 + c] = r[c * nz + j];
+    }
+}
+
+// rows: inverse FFT along x, normalise by 1/(nx nz), unpack the slice pair.
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp, int lognx) {
+    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2];
+    const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
+    const int lines = FFT_PTS / nx;
+    const int64_t row0 = (int64_t)blockIdx.x * lines;
+    make_twiddles(tw, nx, true);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
+    __syncthreads();
+    const float2 *o = lds_fft(b0, b1, tw, nx, lognx);
+    const float norm = 1.0f / (float)(nx * nz);
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+        const int l = e / nx, i = e - l * nx;
+        const int64_t row = row0 + l;
+        const int64_t plane = row / nz, j = row - plane * nz;
+        const int64_t sa = 2 * plane, sb = sa + 1;
+        phi[(sa * nz + j) * nx + i] = o[e].x * norm;
+        if (sb < ns) phi[(sb * nz + j) * nx + i] = o[e].y * norm;
+    }
+}
+
+// compute_field_from_potential_numba_SBS (src/FFAG_SC_new.py:672-728):
+// float32 differences, float64 scaling, float32 store.
+__global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float *Ex, float *Ez, ffa_sc_params sp,
+                                                   const ffa_sc_result *res) {
+    const int nx = sp.nx, nz = sp.nz;
+    const int64_t total = (int64_t)sp.ns * nz * nx;
+    const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (t >= total) return;
+    const int64_t k = t / ((int64_t)nz * nx);
+    const int rem = (int)(t - k * (int64_t)nz * nx);
+    const int j = rem / nx, i = rem - j * nx;
+    const double dr = res->dx, dz = res->dz, bl = res->bunch_length;
+    if (res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0) { Ex[t] = 0.0f; Ez[t] = 0.0f; return; }
+    const double gamma = res->mean_gamma_m1 + 1.0;
+    const double ig2 = 1.0 / (gamma * gamma);
+    const double i2r = 0.5 / dr, i2z = 0.5 / dz;
+    const float *P = phi + k * (int64_t)nz * nx;
+#define PH(jj, ii) P[(jj) * nx + (ii)]
+    double ex, ez;
+    if (i == 0 || i == nx - 1) {
+        const int jp = j + 1 < nz - 1 ? j + 1 : nz - 1, jm = j - 1 > 0 ? j - 1 : 0;
+        ex = i == 0 ? -(double)(PH(j, 1) - PH(j, 0)) / dr * ig2 : -(double)(PH(j, nx - 1) - PH(j, nx - 2)) / dr * ig2;
+        ez = -(double)(PH(jp, i) - PH(jm, i)) / dz * ig2;
+    } else if (j == 0) {
+        ex = -(double)(PH(0, i + 1) - PH(0, i - 1)) * i2r * ig2;
+        ez = -(double)(PH(1, i) - PH(0, i)) / dz * ig2;
+    } else if (j == nz - 1) {
+        ex = -(double)(PH(nz - 1, i + 1) - PH(nz - 1, i - 1)) * i2r * ig2;
+        ez = -(double)(PH(nz - 1, i) - PH(nz - 2, i)) / dz * ig2;
+    } else {
+        ex = -(double)(PH(j, i + 1) - PH(j, i - 1)) * i2r * ig2;
+        ez = -(double)(PH(j + 1, i) - PH(j - 1, i)) * i2z * ig2;
+    }
+#undef PH
+    Ex[t] = (float)ex;
+    Ez[t] = (float)ez;
+}
+
+// interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
+// fused with the endpoint roll of src/FFAG_track.py:356-382.
+template <typename ET>
+__global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
+                                                         const float *Exc, const float *Ezc, const double *px,
+                                                         const double *ppr0, const int32_t *pslice) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    ET *ep = reinterpret_cast<ET *>(p.esc_pre);
+    ET *epp = reinterpret_cast<ET *>(p.esc_prepre);
+    const uint8_t fl = p.flags[i];
+    const bool alive = fl & FFA_F_ALIVE;
+    const double o_r = (double)ep[i], o_z = (double)ep[n + i], o_f = (double)ep[2 * n + i];
+    double nr = o_r, nz_ = o_z, nf = o_f;
+    if (alive) {
+        if (res->bad_grid == 0.0) {
+            const int nx = sp.nx, nz = sp.nz;
+            int s = pslice[i];
+            if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
+            const float *Ex = Exc + (size_t)s * nz * nx, *Ez = Ezc + (size_t)s * nz * nx;
+            const double t = (px[i] - res->xmin) / res->dx, u = (p.z[i] - res->zmin) / res->dz;
+            const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
+            int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cu);
+            if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
+            if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
+            const double al = t - (double)i0, be = u - (double)j0;
+            const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
+            const int64_t b = j0 * nx + i0;
+            const double er = w00 * Ex[b] + w10 * Ex[b + 1] + w01 * Ex[b + nx] + w11 * Ex[b + nx + 1];
+            const double ez = w00 * Ez[b] + w10 * Ez[b + 1] + w01 * Ez[b + nx] + w11 * Ez[b + nx + 1];
+            const double pr = ppr0[i];
+            nr = er * (1.0 / sqrt(1.0 + pr * pr));
+            nz_ = ez;
+            nf = er * (-pr / sqrt(1.0 + pr * pr));
+        } else if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
+            // degenerate grid: the reference leaves LocalBunch columns 10-12 as
+            // the previous step wrote them (the extrapolated field)
+            nr = o_r + sp.s_last * (o_r - (double)epp[i]);
+            nz_ = o_z + sp.s_last * (o_z - (double)epp[n + i]);
+            nf = o_f + sp.s_last * (o_f - (double)epp[2 * n + i]);
+        }
+    }
+    uint8_t nfl = fl;
+    if (sp.first) {
+        const ET a = alive ? (ET)nr : ep[i], b = alive ? (ET)nz_ : ep[n + i], c = alive ? (ET)nf : ep[2 * n + i];
+        ep[i] = a; ep[n + i] = b; ep[2 * n + i] = c;
+        epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
+        nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | (alive ? FFA_F_SCPRE : 0));
+    } else {
+        epp[i] = ep[i]; epp[n + i] = ep[n + i]; epp[2 * n + i] = ep[2 * n + i];
+        if (alive) { ep[i] = (ET)nr; ep[n + i] = (ET)nz_; ep[2 * n + i] = (ET)nf; }
+        nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | ((fl & FFA_F_SCPRE) ? FFA_F_SCPREPRE : 0) |
+                        (alive ? FFA_F_SCPRE : 0));
+    }
+    if (nfl != fl) p.flags[i] = nfl;
+}
+
+inline unsigned gridn(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+inline int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+}  // namespace
+
+int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *L) {
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t C = (size_t)ns * nz * nx;
+    const size_t P = (size_t)((ns + 1) / 2) * nz * nx;
+    size_t o = 0;
+    L->grid = o; o = al(o + 8 * C);
+    L->cell_count = o; o = al(o + 4 * C);
+    L->cbuf = o; o = al(o + 8 * P);
+    L->phi = o; o = al(o + 4 * C);
+    L->ex = o; o = al(o + 4 * C);
+    L->ez = o; o = al(o + 4 * C);
+    L->px = o; o = al(o + 8 * (size_t)n);
+    L->ppr0 = o; o = al(o + 8 * (size_t)n);
+    L->pslice = o; o = al(o + 4 * (size_t)n);
+    L->slice_count = o; o = al(o + 4 * (size_t)ns);
+    L->partials = o; o = al(o + sizeof(double) * 16 * NB);
+    L->result = o; o = al(o + sizeof(ffa_sc_result));
+    L->total = o;
+    return 0;
+}
+
+hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp, void *ws,
+                           const ffa_sc_layout *L) {
+    char *w = reinterpret_cast<char *>(ws);
+    int64_t *grid = reinterpret_cast<int64_t *>(w + L->grid);
+    int32_t *cell = reinterpret_cast<int32_t *>(w + L->cell_count);
+    float2 *cbuf = reinterpret_cast<float2 *>(w + L->cbuf);
+    float *phi = reinterpret_cast<float *>(w + L->phi);
+    float *ex = reinterpret_cast<float *>(w + L->ex);
+    float *ez = reinterpret_cast<float *>(w + L->ez);
+    double *px = reinterpret_cast<double *>(w + L->px);
+    double *ppr0 = reinterpret_cast<double *>(w + L->ppr0);
+    int32_t *pslice = reinterpret_cast<int32_t *>(w + L->pslice);
+    int32_t *slc = reinterpret_cast<int32_t *>(w + L->slice_count);
+    double *part = reinterpret_cast<double *>(w + L->partials);
+    ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(w + L->result);
+    const ffa_machine &m = c->host_machine;
+    const double E0J = 938.2723e6 * 1.60217662e-19;   // src/FFAG_SC_new.py:115
+    const size_t C = (size_t)sp->ns * sp->nz * sp->nx;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_sc_fill, dim3(1024), dim3(BLOCK), 0, st, grid, cell, C, slc, sp->ns);
+    hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, m.c, E0J, part);
+    hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
+    hipLaunchKernelGGL(k_sc_stage2, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, m.c, E0J, res, px, ppr0, pslice, slc, part);
+    hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
+    if (p->n > 0)
+        hipLaunchKernelGGL(k_sc_deposit, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice,
+                           reinterpret_cast<unsigned long long *>(grid), cell);
+    const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
+    const int64_t planes = (sp->ns + 1) / 2;
+    const int64_t rows = planes * sp->nz;
+    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, grid, cbuf, *sp,
+                       res, lognx);
+    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / (FFT_PTS / sp->nz))), dim3(BLOCK), 0, st,
+                       cbuf, *sp, res, lognz);
+    hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, cbuf, phi, *sp,
+                       lognx);
+    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)C)), dim3(BLOCK), 0, st, phi, ex, ez, *sp, res);
+    if (p->n > 0) {
+        if (p->esc_f32)
+            hipLaunchKernelGGL(k_sc_gather_roll<float>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, ex, ez,
+                               px, ppr0, pslice);
+        else
+            hipLaunchKernelGGL(k_sc_gather_roll<double>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, ex, ez,
+                               px, ppr0, pslice);
+    }
+    return hipGetLastError();
+}

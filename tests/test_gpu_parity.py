@@ -1,0 +1,197 @@
+"""GPU parity of the tracking hot path against the FP64 CPU oracle.
+
+Every test calls the HIP kernels through the C ABI (via the product's Python
+classes) and compares with the oracle on identical seeded inputs."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import cases
+
+DT = 0.05e-9
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_run(M, gp, *, n_steps, t0=0.0, solver="boris", precision="fp64", sc=None, kind="stable"):
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    grid = sc["grid"] if sc else (16, 32, 32, 4.0)
+    B = FFAG_Bunch(M, marcosize=sc["marcosize"] if sc else 1.0, sc_grid_size=grid,
+                   BunchType="Boris" if solver == "boris" else "RK4", precision=precision)
+    cases.set_bunch_seo(B, kind)
+    lp = dict(stop_condition=dict(max_stepsN=n_steps, max_turn=1e9, max_time=1.0), time_step=DT,
+              step_dumps=None, enable_SC=bool(sc), SC_type=1, SC_step=sc["step"] if sc else 20,
+              precision=precision, verbose=False)
+    rk = FFAG_RungeKutta()
+    if solver == "boris":
+        rk.rk4_solve_dt_bunch3_withSC_boris(None, t0, B, gp, lp)
+    else:
+        rk.rk4_solve_dt_bunch3_withSC(None, t0, B, gp, lp)
+    return B
+
+
+VARIANTS = {
+    "plain": dict(),
+    "rf": dict(rf=dict(v0=300e3)),
+    "aperture": dict(aperture=(8.30, 8.70, -0.004, 0.004)),
+    "harmonics": dict(harmonics=[dict(m=1, amp_Gs=20.0, phase_deg=30.0, rmin=8.0, rmax=9.3),
+                                 dict(m=3, amp_Gs=5.0, phase_deg=-45.0, rmin=8.4, rmax=8.6)]),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_boris_f64_one_step(torch_cuda, variant, tmp_path):
+    print(f"[case] boris one step, prescribed lattice, variant={variant}", flush=True)
+    mc, gp = cases.machines("prescribed", max_order=2, tmpdir=str(tmp_path), **VARIANTS[variant])
+    M = cases.bunch("prescribed", n=4096, seed=1)
+    ref = O.Tracker(mc, M.copy())
+    ref.run(0.0, DT, 1)
+    B = _gpu_run(M, gp, n_steps=1, kind="prescribed")
+    got = B.LocalBunch
+    err = cases.rel_err(got, ref.M)
+    print(variant, err, flush=True)
+    assert np.array_equal(got[:, 8], ref.M[:, 8])
+    assert max(err.values()) < 1e-12, err
+
+
+def test_boris_f64_steps_with_injection(torch_cuda, tmp_path):
+    """Staggered injection times: particles enter over the run."""
+    print("[case] boris 400 steps, staggered injection", flush=True)
+    mc, gp = cases.machines("prescribed", max_order=2, rf=dict(v0=300e3), tmpdir=str(tmp_path))
+    M = cases.bunch("prescribed", n=2048, seed=2)
+    M[:, 6] = np.sort(np.random.default_rng(5).uniform(0.0, 15.0, M.shape[0]))   # ns
+    ref = O.Tracker(mc, M.copy())
+    ref.run(0.0, DT, 400)
+    B = _gpu_run(M, gp, n_steps=400, kind="prescribed")
+    got = B.LocalBunch
+    assert np.array_equal(got[:, 7], ref.M[:, 7])
+    err = cases.rel_err(got, ref.M)
+    print(err, flush=True)
+    assert max(err.values()) < 1e-10, err
+
+
+def test_boris_f64_100_turns(torch_cuda, tmp_path):
+    """1e-8 relative after 100 turns (stable lattice, RF on)."""
+    print("[case] boris 100 turns, N=32", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=str(tmp_path))
+    M = cases.bunch("stable", n=32, seed=4)
+    T0 = cases.seo("stable").T0[3]
+    nsteps = int(round(100 * T0 / DT))
+    O.lib().ora_set_threads(1)
+    ref = M.copy()
+    O.run_c(mc, ref, 0.0, DT, nsteps)
+    B = _gpu_run(M, gp, n_steps=nsteps)
+    got = B.LocalBunch
+    turns = got[:, 4].mean() / (2 * np.pi)
+    err = cases.rel_err(got, ref)
+    print("steps", nsteps, "turns", turns, err, flush=True)
+    assert turns > 99
+    assert max(err.values()) < 1e-8, err
+
+
+def test_rk4_f64_parity(torch_cuda, tmp_path):
+    print("[case] rk4 one step and 300 steps", flush=True)
+    mc, gp = cases.machines("prescribed", max_order=2, rf=dict(v0=300e3), tmpdir=str(tmp_path))
+    M = cases.bunch("prescribed", n=2048, seed=6, solver="rk4")
+    for nsteps, tol in ((1, 1e-12), (300, 1e-10)):
+        ref = M.copy()
+        O.lib().ora_set_threads(8)
+        O.run_c(mc, ref, 0.0, DT, nsteps, rk4=True)
+        B = _gpu_run(M.copy(), gp, n_steps=nsteps, solver="rk4", kind="prescribed")
+        err = cases.rel_err(B.LocalBunch, ref)
+        print(nsteps, err, flush=True)
+        assert max(err.values()) < tol, (nsteps, err)
+
+
+@pytest.mark.parametrize("grid", [(32, 64, 64), (33, 32, 128)])
+def test_sc_solve_parity(torch_cuda, grid):
+    """One SC solve: scalars, bit-exact fixed-point grid and per-cell counts,
+    potential/fields and particle kicks."""
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    ns, nz, nx = grid
+    print(f"[case] sc solve grid={grid}", flush=True)
+    n = 20000
+    M = cases.bunch("stable", n=n, seed=3, fi_spread=0.6)
+    M[:, 7] = 1.0
+    M[:, 8] = 1.0
+    M[::97, 8] = 0.0          # some lost particles
+    marcosize = 1e13 / n
+    B = FFAG_Bunch(M, marcosize=marcosize, sc_grid_size=(ns, nx, nz, 4.0), BunchType="Boris")
+    cases.set_bunch_seo(B)
+    B.sc_solve_device(first=True, s_last=0.0)
+    res = B.sc_result()
+    seo = cases.oracle_seo()
+    alive = M[:, 8] != 0
+    ref = O.sc_solve(M.copy(), alive, seo, ns, nx, nz, 4.0, marcosize)
+    assert res["count"] == ref["count"]
+    assert abs(res["fi_s"] - ref["fi_s"]) < 1e-15 and abs(res["fi_e"] - ref["fi_e"]) < 1e-15
+    np.testing.assert_allclose(res["mean_gamma_m1"], ref["mean_gamma_m1"], rtol=1e-12)
+    for k, j in (("sum_x", 0), ("sum_z", 1), ("sum_x2", 2), ("sum_z2", 3)):
+        np.testing.assert_allclose(res[k], ref["sums"][j], rtol=1e-11, atol=1e-17)
+    for k in ("xmin", "xmax", "zmin", "zmax", "bunch_length"):
+        np.testing.assert_allclose(res[k], ref[k], rtol=1e-12)
+    np.testing.assert_array_equal(B.sc_view("slice_count").cpu().numpy(), ref["counts"])
+    # deposit with the GPU's own grid box: integer grid and counts bit-exact
+    bounds = (res["xmin"], res["xmax"], res["zmin"], res["zmax"])
+    M2 = M.copy()
+    ref2 = O.sc_solve(M2, alive, seo, ns, nx, nz, 4.0, marcosize, bounds_override=bounds)
+    g = B.sc_view("grid").cpu().numpy()
+    np.testing.assert_array_equal(g, ref2["grid_fixed"])
+    np.testing.assert_array_equal(B.sc_view("cell_count").cpu().numpy(), ref2["cell_count"])
+    assert g.sum() == int(ref["count"]) * (1 << (2 * ref2["bits"]))
+    phi = B.sc_view("phi").cpu().numpy().astype(np.float64)
+    ex = B.sc_view("ex").cpu().numpy().astype(np.float64)
+    ez = B.sc_view("ez").cpu().numpy().astype(np.float64)
+    for name, a, b in (("phi", phi, ref2["phi"]), ("Ex", ex, ref2["Ex"]), ("Ez", ez, ref2["Ez"])):
+        e = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(name, e, flush=True)
+        assert e < 2e-5, (name, e)
+    got = B.esc_pre.cpu().numpy().T
+    want = M2[:, 10:13]
+    scale = np.max(np.abs(want[alive]), axis=0)
+    e = np.max(np.abs(got[alive] - want[alive]) / scale)
+    print("kick", e, flush=True)
+    assert e < 2e-5
+    # endpoint roll: first solve copies Pre into PrePre; flags set for survivors
+    assert np.array_equal(B.esc_prepre.cpu().numpy(), B.esc_pre.cpu().numpy())
+    fl = B.flags.cpu().numpy()
+    assert np.array_equal((fl & 4) != 0, alive) and not np.any(fl & 8)
+
+
+def test_track_with_sc_parity(torch_cuda, tmp_path):
+    """The full loop with SC every 20 steps against the oracle loop."""
+    print("[case] loop with SC, 100 steps", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=str(tmp_path))
+    n = 4000
+    M = cases.bunch("stable", n=n, seed=8, fi_spread=1.0)
+    sc = dict(grid=(16, 32, 32, 4.0), marcosize=1e13 / n, step=20)
+    ref = O.Tracker(mc, M.copy(), enable_sc=True, sc_step=20, grid=(16, 32, 32, 4.0), marcosize=1e13 / n,
+                    seo=cases.oracle_seo())
+    ref.run(0.0, DT, 100)
+    B = _gpu_run(M, gp, n_steps=100, sc=sc)
+    err = cases.rel_err(B.LocalBunch, ref.M)
+    print(err, flush=True)
+    assert max(err.values()) < 1e-9, err
+    # the kicks themselves (FP32 grid) agree to FP32 accuracy
+    got = B.LocalBunch[:, 10:13]
+    want = ref.M[:, 10:13]
+    e = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert e < 1e-4, e
+
+
+def test_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
+    """The fused FP32-field tier tracks the FP64 path to FP32 field accuracy."""
+    print("[case] fp32 tier vs fp64, 2000 steps with SC", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=str(tmp_path))
+    from pyffa_amd.FFAG_Field import FFAG_Bfield_analytical  # noqa: F401
+    n = 4000
+    M = cases.bunch("stable", n=n, seed=9, fi_spread=1.0)
+    sc = dict(grid=(16, 32, 32, 4.0), marcosize=1e13 / n, step=20)
+    B64 = _gpu_run(M.copy(), gp, n_steps=2000, sc=sc)
+    B32 = _gpu_run(M.copy(), gp, n_steps=2000, sc=sc, precision="fp32")
+    a, b = B32.LocalBunch, B64.LocalBunch
+    err = cases.rel_err(a, b)
+    print(err, flush=True)
+    assert np.array_equal(a[:, 8], b[:, 8])
+    assert err[0] < 1e-6 and err[4] < 1e-6 and err[9] < 1e-9, err
+    assert max(err.values()) < 1e-3, err
