@@ -171,7 +171,9 @@ def test_track_with_sc_parity(torch_cuda, tmp_path):
     B = _gpu_run(M, gp, n_steps=100, sc=sc)
     err = cases.rel_err(B.LocalBunch, ref.M)
     print(err, flush=True)
-    assert max(err.values()) < 1e-9, err
+    # SC fields differ at FP32 level (complex64 FFT vs the oracle's FP64 FFT)
+    assert max(err[0], err[4], err[9]) < 1e-10, err
+    assert max(err.values()) < 1e-6, err
     # the kicks themselves (FP32 grid) agree to FP32 accuracy
     got = B.LocalBunch[:, 10:13]
     want = ref.M[:, 10:13]
