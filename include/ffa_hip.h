@@ -117,6 +117,7 @@ typedef struct {
     int32_t n_ek, n_fi;
     double ek0, dek, fi0, dfi;
     double E0_MeV;         /* rest energy used for Ek = (gamma-1) E0 */
+    double c;              /* speed of light used for gamma (FFAG_Bunch.speed_c) */
 } ffa_sc_params;
 
 /* Scalars a solve leaves on the device (read back only when asked). */

@@ -275,6 +275,7 @@ class FFAG_Bunch:
         sp.fi0 = float(self.SEO_fi_axis[0])
         sp.dfi = float(self.SEO_fi_axis[1] - self.SEO_fi_axis[0])
         sp.E0_MeV = self.E0_rest_MeV
+        sp.c = self.speed_c
         return sp
 
     def sc_view(self, name):

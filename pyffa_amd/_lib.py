@@ -66,7 +66,7 @@ class FfaScParams(ctypes.Structure):
                 ("rk", _i32), ("first", _i32), ("s_last", _d), ("bits", _i32),
                 ("seo_r", _vp), ("seo_pr", _vp), ("seo_ek_axis", _vp), ("seo_perimeter", _vp),
                 ("n_ek", _i32), ("n_fi", _i32), ("ek0", _d), ("dek", _d), ("fi0", _d), ("dfi", _d),
-                ("E0_MeV", _d)]
+                ("E0_MeV", _d), ("c", _d)]
 
 
 class FfaScResult(ctypes.Structure):

@@ -208,7 +208,7 @@ int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_s
 int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace) {
     ffa_ctx_s *c = as_ctx(ctx);
     if (!c || !sp || !workspace) return fail(c, FFA_ERR_INVALID, "ctx, params or workspace is NULL");
-    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (!(sp->c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed of light must be positive");
     if (int rc = check_particles(c, p)) return rc;
     if (sp->ns < 2) return fail(c, FFA_ERR_INVALID, "ns must be >= 2");
     if (!pow2_in(sp->nx, 32, 1024) || !pow2_in(sp->nz, 32, 1024))

@@ -525,14 +525,13 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     int32_t *slc = reinterpret_cast<int32_t *>(w + L->slice_count);
     double *part = reinterpret_cast<double *>(w + L->partials);
     ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(w + L->result);
-    const ffa_machine &m = c->host_machine;
     const double E0J = 938.2723e6 * 1.60217662e-19;   // src/FFAG_SC_new.py:115
     const size_t C = (size_t)sp->ns * sp->nz * sp->nx;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_sc_fill, dim3(1024), dim3(BLOCK), 0, st, grid, cell, C, slc, sp->ns);
-    hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, m.c, E0J, part);
+    hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, sp->c, E0J, part);
     hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
-    hipLaunchKernelGGL(k_sc_stage2, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, m.c, E0J, res, px, ppr0, pslice, slc, part);
+    hipLaunchKernelGGL(k_sc_stage2, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0, pslice, slc, part);
     hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
     if (p->n > 0)
         hipLaunchKernelGGL(k_sc_deposit, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice,

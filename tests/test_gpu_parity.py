@@ -80,13 +80,25 @@ def test_boris_f64_100_turns(torch_cuda, tmp_path):
     O.lib().ora_set_threads(1)
     ref = M.copy()
     O.run_c(mc, ref, 0.0, DT, nsteps)
+    # the problem's own conditioning: the oracle run again from initial r, z
+    # moved by one ulp
+    pert = M.copy()
+    pert[:, 0] = np.nextafter(pert[:, 0], np.inf)
+    pert[:, 2] = np.nextafter(pert[:, 2], np.inf)
+    O.run_c(mc, pert, 0.0, DT, nsteps)
+    sens = cases.rel_err(pert, ref)
     B = _gpu_run(M, gp, n_steps=nsteps)
     got = B.LocalBunch
     turns = got[:, 4].mean() / (2 * np.pi)
     err = cases.rel_err(got, ref)
-    print("steps", nsteps, "turns", turns, err, flush=True)
+    print("steps", nsteps, "turns", turns, "gpu-oracle", err, "1-ulp sensitivity", sens, flush=True)
     assert turns > 99
-    assert max(err.values()) < 1e-8, err
+    # horizontal, azimuthal and RF-phase state: 1e-8 after 100 turns
+    assert max(err[c] for c in (0, 1, 4, 5, 9)) < 1e-8, err
+    # the vertical plane of this lattice amplifies a 1-ulp change ~1e9-fold in
+    # 100 turns (sens ~1e-7), so z, vz are held to the problem's conditioning
+    for c in (2, 3):
+        assert err[c] < max(30 * sens[c], 1e-8), (c, err[c], sens[c])
 
 
 def test_rk4_f64_parity(torch_cuda, tmp_path):
@@ -179,6 +191,22 @@ def test_track_with_sc_parity(torch_cuda, tmp_path):
     want = ref.M[:, 10:13]
     e = np.max(np.abs(got - want)) / np.max(np.abs(want))
     assert e < 1e-4, e
+
+
+def test_fp32_tier_no_sc_close_to_fp64(torch_cuda, tmp_path):
+    """Fused FP32-field tier without SC, 1000 steps in fused blocks, against
+    the FP64 path on the same inputs."""
+    print("[case] fp32 tier vs fp64, 1000 steps, no SC", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=300e3), tmpdir=str(tmp_path))
+    M = cases.bunch("stable", n=4000, seed=10)
+    B64 = _gpu_run(M.copy(), gp, n_steps=1000)
+    B32 = _gpu_run(M.copy(), gp, n_steps=1000, precision="fp32")
+    a, b = B32.LocalBunch, B64.LocalBunch
+    err = cases.rel_err(a, b)
+    print(err, flush=True)
+    assert np.array_equal(a[:, 8], b[:, 8])
+    assert err[0] < 1e-6 and err[4] < 1e-6 and err[9] < 1e-12, err
+    assert max(err.values()) < 1e-3, err
 
 
 def test_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
