@@ -31,18 +31,21 @@ HBM_PEAK_GBS = 8000.0
 def build_inputs(n, tmpdir):
     import numpy as np
     from tests import cases
-    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=tmpdir)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), aperture=cases.APERTURE, tmpdir=tmpdir)
     M = cases.bunch("stable", n=n, seed=0)
     return mc, gp, M
 
 
-def moments(B):
-    """Centroid and rms emittances (r, vr/v) and (z, vz/v) of alive particles."""
+def snapshot(B):
+    return {k: getattr(B, k).clone() for k in ("r", "vr", "z", "vz", "fidot", "flags")}
+
+
+def moments(S, alive):
+    """Centroid and rms emittances (r, vr/v) and (z, vz/v) over the mask."""
     import torch
-    alive = (B.flags & 2) != 0
-    r, z = B.r[alive], B.z[alive]
-    v = torch.sqrt(B.vr[alive] ** 2 + B.vz[alive] ** 2 + (B.r[alive] * B.fidot[alive]) ** 2)
-    xp, zp = B.vr[alive] / v, B.vz[alive] / v
+    r, z = S["r"][alive], S["z"][alive]
+    v = torch.sqrt(S["vr"][alive] ** 2 + S["vz"][alive] ** 2 + (S["r"][alive] * S["fidot"][alive]) ** 2)
+    xp, zp = S["vr"][alive] / v, S["vz"][alive] / v
 
     def emit(a, b):
         a = a - a.mean()
@@ -61,20 +64,20 @@ def run_tier(precision, M, gp, warmup, steps):
     cases.set_bunch_seo(B)
     rk = FFAG_RungeKutta()
 
-    def lp(nmacro):
-        return dict(stop_condition=dict(max_stepsN=nmacro * SC_STEP, max_turn=1e9, max_time=1.0),
+    def lp(first, last):
+        return dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0),
                     time_step=DT, step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP,
-                    precision=precision, verbose=False)
+                    precision=precision, verbose=False, start_step=first * SC_STEP)
     t = 0.0
     if warmup:
-        rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(warmup))
+        rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(0, warmup))
         t = rk.last_run["t"]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(steps))
+    rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(warmup, warmup + steps))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return B, rk, el
+    return B, rk, el, snapshot(B)
 
 
 def kernel_time_ms(rk, B, gp, precision, reps=10):
@@ -153,8 +156,7 @@ def main():
     tmp = tempfile.mkdtemp()
     mc, gp, M = build_inputs(args.particles, tmp)
     print(f"[bench] inputs ready: {M.shape[0]} particles", file=sys.stderr, flush=True)
-    B32, rk32, el32 = run_tier("fp32", M, gp, args.warmup, args.steps)
-    m32 = moments(B32)
+    B32, rk32, el32, S32 = run_tier("fp32", M, gp, args.warmup, args.steps)
     k32 = kernel_time_ms(rk32, B32, gp, "fp32")
     print(f"[bench] fp32 tier {el32:.3f} s, kernels {k32}", file=sys.stderr, flush=True)
     n_alive = M.shape[0]
@@ -164,14 +166,15 @@ def main():
         "value": ps / el32, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el32 / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 state / f32 field", "data": "synthetic",
-        "config": {"workload": "cfg1: 1M protons, 12-sector scaling FFAG r0=8.5 m k=7.6 (D=-1.5 stable variant), "
-                               "order-2 Taylor maps, 2.5D SC (ns,nx,nz)=(256,128,128) every 20 steps, 2 RF gaps "
+        "config": {"workload": "cfg1: 1M protons (1e13 total), 12-sector scaling FFAG r0=8.5 m k=7.6 "
+                               "(D=-1.4 vertically stable variant), order-2 Taylor maps, aperture r 8.0-9.3 m |z|<5 cm, 2.5D SC (ns,nx,nz)=(256,128,128) every 20 steps, 2 RF gaps "
                                "100 kV, Boris dt=0.05 ns; FP32-field tier",
                    "particles": n_alive, "sc_grid": list(GRID[:3]), "sc_step": SC_STEP},
     }
     if not args.no_fp64:
-        B64, rk64, el64 = run_tier("fp64", M, gp, args.warmup, args.steps)
-        m64 = moments(B64)
+        B64, rk64, el64, S64 = run_tier("fp64", M, gp, args.warmup, args.steps)
+        both = ((S32["flags"] & 2) != 0) & ((S64["flags"] & 2) != 0)
+        m32, m64 = moments(S32, both), moments(S64, both)
         k64 = kernel_time_ms(rk64, B64, gp, "fp64")
         print(f"[bench] fp64 path {el64:.3f} s, kernels {k64}", file=sys.stderr, flush=True)
         out["fp64_value"] = ps / el64
@@ -181,7 +184,8 @@ def main():
             "centroid_z_abs_m": abs(m32["z_mean"] - m64["z_mean"]),
             "emit_r_rel": abs(m32["emit_r"] / m64["emit_r"] - 1.0),
             "emit_z_rel": abs(m32["emit_z"] / m64["emit_z"] - 1.0),
-            "alive_fp32": m32["alive"], "alive_fp64": m64["alive"]}
+            "alive_fp32": int(((S32["flags"] & 2) != 0).sum()), "alive_fp64": int(((S64["flags"] & 2) != 0).sum()),
+            "compared": m32["alive"], "after_steps": (args.warmup + args.steps) * SC_STEP}
         out["fp64_kernel_ms"] = k64
     # roofline of the dominant kernel of the headline tier: the fused 20-step push.
     # Algorithmic HBM bytes per particle per launch: FP64 state read + write

@@ -386,9 +386,10 @@ class Tracker:
             self.head = e
             self.injected += b - a
 
-    def run(self, t_start, dt, nsteps, max_turn=None, max_time=None):
+    def run(self, t_start, dt, nsteps, max_turn=None, max_time=None, start_step=0):
+        """Steps start_step .. nsteps-1 (start_step > 0 continues a run)."""
         t = t_start
-        steps_i = 0
+        steps_i = start_step
         M = self.M
         while steps_i < nsteps:
             self.inject(t, dt)

@@ -13,7 +13,10 @@ torch stream, synchronising only for the progress line and the stop check.
 
 ``LocalParameters`` accepts, besides the reference keys, ``precision``
 ('fp64' default, 'fp32' for the FP32-field tier, which runs the steps
-between two SC solves as one fused launch) and ``verbose`` (progress lines).
+between two SC solves as one fused launch), ``verbose`` (progress lines),
+``start_step`` (continue a run: the step index the loop starts from, so the
+SC schedule and endpoint history carry on; ``max_stepsN`` stays the absolute
+end) and ``max_block`` (cap on the steps issued per launch block).
 """
 from __future__ import annotations
 
@@ -132,7 +135,8 @@ class FFAG_RungeKutta:
         lib = ctx.lib
         has_dumps = len(getattr(dumps, "dumps", [1])) > 0
         t_pre = t_start
-        steps_i = 0
+        steps_i = int(LocalParameters.get("start_step", 0))
+        max_block = int(LocalParameters.get("max_block", _MAX_BLOCK))
         fused = precision == "fp32" and solver == "boris"
         n_solves = 0
         while steps_i < max_steps:
@@ -143,7 +147,7 @@ class FFAG_RungeKutta:
             nb = 1000 - steps_i % 1000
             if enable_sc:
                 nb = min(nb, sc_step - steps_i % sc_step)
-            nb = min(nb, max_steps - steps_i, _MAX_BLOCK)
+            nb = min(nb, max_steps - steps_i, max(1, min(max_block, _MAX_BLOCK)))
             if has_dumps:
                 nb = 1
             b0, e0 = BunchObj.injection_range(t_pre, step_t)

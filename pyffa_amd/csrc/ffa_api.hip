@@ -199,7 +199,7 @@ int ffa_push_boris_fused_f32(void *ctx, const ffa_particles *p, const ffa_step *
 
 int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *out) {
     if (!out) return fail(nullptr, FFA_ERR_INVALID, "out is NULL");
-    if (n < 0 || ns < 2) return fail(nullptr, FFA_ERR_INVALID, "need n >= 0 and ns >= 2");
+    if (n < 0 || ns < 2 || ns > 4096) return fail(nullptr, FFA_ERR_INVALID, "need n >= 0 and 2 <= ns <= 4096");
     if (!pow2_in(nx, 32, 1024) || !pow2_in(nz, 32, 1024))
         return fail(nullptr, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", nx, nz);
     return ffa_sc_layout_compute(n, ns, nz, nx, out);
@@ -210,7 +210,7 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
     if (!c || !sp || !workspace) return fail(c, FFA_ERR_INVALID, "ctx, params or workspace is NULL");
     if (!(sp->c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed of light must be positive");
     if (int rc = check_particles(c, p)) return rc;
-    if (sp->ns < 2) return fail(c, FFA_ERR_INVALID, "ns must be >= 2");
+    if (sp->ns < 2 || sp->ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [2, 4096]");
     if (!pow2_in(sp->nx, 32, 1024) || !pow2_in(sp->nz, 32, 1024))
         return fail(c, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", sp->nx, sp->nz);
     if (sp->bits < 1 || sp->bits > 30) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 30]");

@@ -164,10 +164,17 @@ __device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, 
 }
 
 // SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
+// Slice histogram: per-block counts in LDS, one global atomic per non-empty
+// (block, slice) pair (ns <= FFA_SC_MAX_NS).
+constexpr int MAX_NS_LDS = 4096;
+
 __global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_params sp, double c, double E0J,
                                                     const ffa_sc_result *res, double *px, double *ppr0,
                                                     int32_t *pslice, int32_t *slc, double *part) {
     __shared__ double sh[4][BLOCK];
+    __shared__ int32_t hist[MAX_NS_LDS];
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
+    __syncthreads();
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const double fs = res->fi_s, step = res->slice_step;
     for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
@@ -188,10 +195,12 @@ __global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_par
         px[i] = x;
         ppr0[i] = pr0;
         pslice[i] = (int32_t)s;
-        atomicAdd(&slc[s], 1);
+        atomicAdd(&hist[s], 1);
     }
     const int op[4] = {0, 0, 0, 0};
-    block_reduce<4>(sh, v, op);
+    block_reduce<4>(sh, v, op);       // contains __syncthreads: hist is complete
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK)
+        if (hist[k]) atomicAdd(&slc[k], hist[k]);
     if (threadIdx.x == 0)
         for (int k = 0; k < 4; ++k) part[16 * blockIdx.x + 10 + k] = v[k];
 }

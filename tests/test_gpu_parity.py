@@ -171,55 +171,66 @@ def test_sc_solve_parity(torch_cuda, grid):
 
 
 def test_track_with_sc_parity(torch_cuda, tmp_path):
-    """The full loop with SC every 20 steps against the oracle loop.
-
-    The reference slices [phi_s, phi_e] with step (phi_e - phi_s)/(ns - 1)
-    into ns slices, so the particle that defines phi_e sits alone in the last
-    slice or joins the one before depending on the last ulp of
-    (phi - phi_s)/step; a 1e-13 state difference can flip it, its kick
-    changes completely and, through the next solve, every particle's kick
-    shifts slightly.  For this seed no flip happens in the first 60 steps:
-    per-particle parity is checked there, bunch moments at 100 steps."""
-    print("[case] loop with SC, 59 and 100 steps", flush=True)
-    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=str(tmp_path))
+    """The full loop with SC every 20 steps, checked one macro step (20 pushes
+    + 1 solve) at a time: before each macro step the oracle is set to the GPU
+    state (including the SC endpoint history), so both see bit-identical
+    inputs and the check measures one macro step's error, not accumulated
+    drift.  (Without the resync the particle that defines phi_e can flip
+    between the last two slices -- the reference slices [phi_s, phi_e] in
+    steps of (phi_e - phi_s)/(ns - 1) -- which changes its kick entirely.)"""
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    print("[case] loop with SC, 6 macro steps, resynchronised", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), aperture=cases.APERTURE, tmpdir=str(tmp_path))
     n = 4000
     M = cases.bunch("stable", n=n, seed=8, fi_spread=1.0)
-    sc = dict(grid=(16, 32, 32, 4.0), marcosize=1e11 / n, step=20)
-    for nsteps in (59, 100):
-        ref = O.Tracker(mc, M.copy(), enable_sc=True, sc_step=20, grid=(16, 32, 32, 4.0), marcosize=1e11 / n,
-                        seo=cases.oracle_seo())
-        ref.run(0.0, DT, nsteps)
-        B = _gpu_run(M.copy(), gp, n_steps=nsteps, sc=sc)
-        a, b = B.LocalBunch, ref.M
-        if nsteps == 59:
-            err = cases.rel_err(a, b)
-            print(nsteps, err, flush=True)
-            # r, phi, RF phase: FP64 round-off; z, vz carry the FP32 grid's rounding
-            assert max(err[0], err[4], err[9]) < 1e-10, err
-            assert max(err.values()) < 1e-6, err
-            e = np.max(np.abs(a[:, 10:13] - b[:, 10:13])) / np.max(np.abs(b[:, 10:13]))
-            assert e < 1e-4, e
-        else:
-            ma, mb = _moments(a), _moments(b)
-            print(nsteps, ma, mb, flush=True)
-            assert abs(ma["r"] - mb["r"]) < 1e-9 and abs(ma["z"] - mb["z"]) < 1e-9
-            assert abs(ma["er"] / mb["er"] - 1) < 1e-6 and abs(ma["ez"] / mb["ez"] - 1) < 1e-6
+    grid = (16, 32, 32, 4.0)
+    marc = 1e13 / n
+    B = FFAG_Bunch(M.copy(), marcosize=marc, sc_grid_size=grid, BunchType="Boris")
+    cases.set_bunch_seo(B)
+    rk = FFAG_RungeKutta()
+    tr = O.Tracker(mc, M.copy(), enable_sc=True, sc_step=20, grid=grid, marcosize=marc, seo=cases.oracle_seo())
+    t = 0.0
+    for blk in range(6):
+        s0 = 20 * blk
+        if blk > 0:
+            cases.sync_tracker_from_bunch(tr, B)
+        tr.run(t, DT, s0 + 20, start_step=s0)
+        lp = dict(stop_condition=dict(max_stepsN=s0 + 20, max_turn=1e9, max_time=1.0), time_step=DT,
+                  step_dumps=None, enable_SC=True, SC_type=1, SC_step=20, verbose=False, start_step=s0)
+        rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
+        t = rk.last_run["t"]
+        a, b = B.LocalBunch, tr.M
+        err = cases.rel_err(a, b)
+        ek = np.max(np.abs(B.esc_pre.cpu().numpy().T - tr.E_pre)) / np.max(np.abs(tr.E_pre))
+        print(blk, err, "kick", ek, flush=True)
+        assert np.array_equal(a[:, 8], b[:, 8])
+        # r, phi, RF phase: FP64 round-off over 20 steps; z, vz carry the FP32
+        # grid's rounding through the kick
+        assert max(err[0], err[4], err[9]) < 1e-11, (blk, err)
+        assert max(err.values()) < 1e-6, (blk, err)
+        assert ek < 2e-5, (blk, ek)
 
 
 def test_fp32_tier_no_sc_close_to_fp64(torch_cuda, tmp_path):
     """Fused FP32-field tier without SC, 1000 steps in fused blocks, against
     the FP64 path on the same inputs."""
     print("[case] fp32 tier vs fp64, 1000 steps, no SC", flush=True)
-    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=300e3), tmpdir=str(tmp_path))
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=300e3), aperture=cases.APERTURE,
+                            tmpdir=str(tmp_path))
     M = cases.bunch("stable", n=4000, seed=10)
     B64 = _gpu_run(M.copy(), gp, n_steps=1000)
     B32 = _gpu_run(M.copy(), gp, n_steps=1000, precision="fp32")
     a, b = B32.LocalBunch, B64.LocalBunch
-    err = cases.rel_err(a, b)
-    print(err, flush=True)
-    assert np.array_equal(a[:, 8], b[:, 8])
-    assert err[0] < 1e-6 and err[4] < 1e-6 and err[9] < 1e-12, err
-    assert max(err.values()) < 1e-3, err
+    both = (a[:, 8] != 0) & (b[:, 8] != 0)
+    assert np.mean(a[:, 8] != b[:, 8]) < 0.005
+    a, b = a[both], b[both]
+    assert np.array_equal(a[:, 9], b[:, 9])          # FP64 RF phase, same increments
+    for c, tol in ((0, 1e-6), (4, 1e-6), (1, 1e-3), (3, 1e-2)):
+        scale = np.abs(b[:, c]) + np.sqrt(np.mean(b[:, c] ** 2))
+        e = np.abs(a[:, c] - b[:, c]) / scale
+        print(c, "median", np.median(e), "p99", np.quantile(e, 0.99), flush=True)
+        assert np.quantile(e, 0.99) < tol, (c, np.quantile(e, 0.99))
 
 
 def _moments(M):
@@ -238,14 +249,17 @@ def test_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
     """The fused FP32-field tier against the FP64 path with SC over 2000
     steps: bunch centroid and rms emittances agree to < 1e-3 relative."""
     print("[case] fp32 tier vs fp64, 2000 steps with SC", flush=True)
-    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), tmpdir=str(tmp_path))
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), aperture=cases.APERTURE,
+                            tmpdir=str(tmp_path))
     n = 4000
     M = cases.bunch("stable", n=n, seed=9, fi_spread=1.0)
     sc = dict(grid=(16, 32, 32, 4.0), marcosize=1e11 / n, step=20)
     B64 = _gpu_run(M.copy(), gp, n_steps=2000, sc=sc)
     B32 = _gpu_run(M.copy(), gp, n_steps=2000, sc=sc, precision="fp32")
     a, b = B32.LocalBunch, B64.LocalBunch
-    assert np.array_equal(a[:, 8], b[:, 8])
+    assert np.mean(a[:, 8] != b[:, 8]) < 0.005
+    both = (a[:, 8] != 0) & (b[:, 8] != 0)
+    a, b = a[both], b[both]
     ma, mb = _moments(a), _moments(b)
     print(ma, mb, flush=True)
     assert abs(ma["r"] - mb["r"]) < 1e-7 and abs(ma["z"] - mb["z"]) < 1e-7
