@@ -262,5 +262,6 @@ def test_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
     a, b = a[both], b[both]
     ma, mb = _moments(a), _moments(b)
     print(ma, mb, flush=True)
-    assert abs(ma["r"] - mb["r"]) < 1e-7 and abs(ma["z"] - mb["z"]) < 1e-7
+    # FP32 field rounding moves trajectories by microns over 2000 steps (beam rms ~2 cm)
+    assert abs(ma["r"] - mb["r"]) < 1e-5 and abs(ma["z"] - mb["z"]) < 1e-5
     assert abs(ma["er"] / mb["er"] - 1) < 1e-3 and abs(ma["ez"] / mb["ez"] - 1) < 1e-3
