@@ -3,6 +3,7 @@
 // exceptions never cross the boundary.
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include "ffa_internal.h"
@@ -111,9 +112,15 @@ int ffa_set_machine(void *ctx, const ffa_machine *m) {
     }
     if (m->rf_enable && m->n_gap > 0 && !(m->acc_etot == m->acc_etot))
         return fail(c, FFA_ERR_INVALID, "acc_etot is NaN");
+    static_assert(sizeof(ffa_machine) <= FFA_CTX_FCONST_OFF, "machine block overflows");
+    static_assert(FFA_CTX_FCONST_OFF + sizeof(ffa_fconst) <= FFA_CTX_STEPS_OFF, "fconst block overflows");
+    ffa_fconst fc;
+    ffa_fconst_from_machine(*m, fc);
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->dev + FFA_CTX_MACHINE_OFF, m, sizeof(ffa_machine), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dev + FFA_CTX_FCONST_OFF, &fc, sizeof(ffa_fconst), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_check(c, e, "ffa_set_machine");
     c->host_machine = *m;
@@ -232,3 +239,50 @@ int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {
 }
 
 }  // extern "C"
+
+void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {
+    std::memset(&f, 0, sizeof f);
+    f.T = m.taylor_f32 ? reinterpret_cast<const float *>(m.taylor) : nullptr;
+    f.P = (float)(FFA_TWO_PI / (double)m.nsectors);
+    f.invP = (float)((double)m.nsectors / FFA_TWO_PI);
+    f.B0 = (float)m.B0;
+    f.kv = (float)m.k_value;
+    f.n_seg = m.n_seg;
+    for (int j = 0; j < m.n_seg; ++j) {
+        f.seg[j][0] = (float)m.convex[j][0];
+        f.seg[j][1] = (float)m.convex[j][1];
+        f.seg[j][2] = (float)(16.0 / m.convex[j][2]);
+        f.seg[j][3] = (float)(m.convex[j][4] * std::fabs(m.convex[j][3]));
+    }
+    f.n_sp = m.n_spoly;
+    f.n_spp = m.n_sppoly;
+    for (int a = 0; a < m.n_spoly; ++a) f.sp[a] = (float)m.s_poly[a];
+    for (int a = 0; a < m.n_sppoly; ++a) f.spp[a] = (float)m.sp_poly[a];
+    f.no = m.max_order;
+    f.NX = m.r_size;
+    f.NY = m.fi_size;
+    f.r_min = (float)m.r_min;
+    f.inv_rstep = m.r_step > 0.0 ? (float)(1.0 / m.r_step) : 0.0f;
+    f.fi_min = (float)m.fi_min;
+    f.inv_fistep = m.fi_step > 0.0 ? (float)(1.0 / m.fi_step) : 0.0f;
+    f.n_harm = m.n_harm;
+    for (int k = 0; k < m.n_harm; ++k) {
+        f.harm[k][0] = (float)m.harm_m[k];
+        f.harm[k][1] = (float)m.harm_amp[k];
+        f.harm[k][2] = (float)m.harm_phase[k];
+        f.harm[k][3] = (float)m.harm_rmin[k];
+        f.harm[k][4] = (float)m.harm_rmax[k];
+    }
+    f.rf = m.rf_enable;
+    f.n_gap = m.n_gap;
+    f.acc = (float)m.acc_etot;
+    for (int g = 0; g < m.n_gap; ++g) {
+        for (int j = 0; j < 4; ++j) {
+            f.gap_box[g][j] = (float)m.gap_box[g][j];
+            f.gap_px[g][j] = (float)m.gap_px[g][j];
+            f.gap_py[g][j] = (float)m.gap_py[g][j];
+        }
+        f.gap_thn[g] = (float)(m.gap_theta[g] + FFA_PI / 2.0);
+    }
+    for (int g = 0; g < FFA_MAX_GAP; ++g) f.rf_shift[g] = (float)m.rf_shift[g];
+}

@@ -8,6 +8,29 @@
 #define FFA_PI 3.141592653589793
 #define FFA_TWO_PI (2.0 * FFA_PI)
 
+// FP32 constants of the field evaluation, precomputed on the host from the
+// machine description (divisions done once) and read by the FP32 tier with
+// uniform loads.
+struct ffa_fconst {
+    const float *T;                // FP32 interleaved Taylor map (device)
+    float P, invP;                 // sector period, 1 / period
+    float B0, kv;
+    int32_t n_seg, n_sp, n_spp, no;
+    float seg[FFA_MAX_SEG][4];     // x0, x1, 16/W0, y02 |y01|
+    float sp[FFA_MAX_POLY], spp[FFA_MAX_POLY];
+    int32_t NX, NY;
+    float r_min, inv_rstep, fi_min, inv_fistep;
+    int32_t n_harm, rf, n_gap, pad;
+    float harm[FFA_MAX_HARM][5];   // m, amp, phase, rmin, rmax
+    float acc;                     // 1 / gap_width
+    float gap_box[FFA_MAX_GAP][4];
+    float gap_px[FFA_MAX_GAP][4], gap_py[FFA_MAX_GAP][4];
+    float gap_thn[FFA_MAX_GAP];    // gap azimuth + pi/2
+    float rf_shift[FFA_MAX_GAP];
+};
+
+void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f);
+
 struct ffa_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -33,8 +56,14 @@ static constexpr size_t FFA_CTX_RED_BYTES = 64 * 1024;
 static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_RED_OFF + FFA_CTX_RED_BYTES;
 static constexpr int FFA_MAX_FUSED = (int)(FFA_CTX_STEPS_BYTES / sizeof(ffa_step));
 
+static constexpr size_t FFA_CTX_FCONST_OFF = 6144;   // after the machine (< 6 KB)
+
 inline const ffa_machine *ctx_machine_dev(ffa_ctx_s *c) {
     return reinterpret_cast<const ffa_machine *>(c->dev + FFA_CTX_MACHINE_OFF);
+}
+
+inline const ffa_fconst *ctx_fconst_dev(ffa_ctx_s *c) {
+    return reinterpret_cast<const ffa_fconst *>(c->dev + FFA_CTX_FCONST_OFF);
 }
 
 // launchers implemented in the .hip files; return a hipError_t
