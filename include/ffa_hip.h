@@ -110,7 +110,7 @@ typedef struct {
     int32_t rk;            /* gamma from E_total (GetLocalCoordinates_2p5_rk) */
     int32_t first;         /* first solve of the run (steps_i == 0) */
     double s_last;         /* extrapolation factor of the step before this solve */
-    int32_t bits;          /* fixed-point fractional bits per CIC axis */
+    int32_t bits;          /* fixed-point fractional bits per CIC axis, 1..20 */
     /* SEO tables, device, row-major (n_ek, n_fi) */
     const double *seo_r, *seo_pr;
     const double *seo_ek_axis, *seo_perimeter;   /* n_ek each */
@@ -146,6 +146,9 @@ typedef struct {
     size_t slice_count; /* int32 ns */
     size_t partials;    /* double [FFA_SC_BLOCKS][16] */
     size_t result;      /* ffa_sc_result */
+    size_t records;     /* uint64 n: slice-sorted packed CIC records */
+    size_t slice_start; /* int32 ns + 1: record offset of each slice */
+    size_t cursor;      /* int32 ns: fill cursors */
 } ffa_sc_layout;
 
 #define FFA_SC_BLOCKS 1024

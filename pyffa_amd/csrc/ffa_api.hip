@@ -220,7 +220,7 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
     if (sp->ns < 2 || sp->ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [2, 4096]");
     if (!pow2_in(sp->nx, 32, 1024) || !pow2_in(sp->nz, 32, 1024))
         return fail(c, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", sp->nx, sp->nz);
-    if (sp->bits < 1 || sp->bits > 30) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 30]");
+    if (sp->bits < 1 || sp->bits > 20) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 20]");
     if ((double)p->n * (double)(1LL << (2 * sp->bits)) >= 9.2e18)
         return fail(c, FFA_ERR_INVALID, "fixed-point scale 2^%d too large for %lld particles", 2 * sp->bits,
                     (long long)p->n);

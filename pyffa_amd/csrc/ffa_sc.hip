@@ -1,16 +1,21 @@
 // 2.5D space-charge solve of the tracking hot path (gfx950).
 //
 // Pipeline of one solve (every kernel on the context stream, no host sync):
-//   k_sc_fill          zero the int64 grid, per-cell and per-slice counters
+//   k_sc_fill          zero the per-slice counters
 //   k_sc_stage1        per survivor: gamma, phi mod 2pi; per-block partial
 //                      sums of gamma and the four covering-arc min/max
 //   k_sc_final1        1 block: covering arc, slice step, mean gamma,
 //                      perimeter, bunch length (device-side scalars)
 //   k_sc_stage2        per survivor: SEO r0/pr0, x = r - r0, slice index,
 //                      per-block partial moments, slice histogram
-//   k_sc_final2        1 block: grid box, dx, dz, charge scale
-//   k_sc_deposit       fixed-point CIC deposit with int64 atomics (integer
-//                      sums: bit-identical for any particle order)
+//   k_sc_final2        1 block: grid box, dx, dz, charge scale, slice offsets
+//   k_sc_bin           per survivor: CIC cell + fixed-point weights packed in
+//                      one 64-bit record, written slice-sorted (two-level
+//                      reservation: one global atomic per block and slice)
+//   k_sc_deposit_tiles one workgroup per (slice, z band): LDS int64 / int32
+//                      accumulation of the slice's records, plain stores of
+//                      the band (integer sums: bit-identical for any order,
+//                      and no device-scope atomics on the grid)
 //   k_fft_rows_fwd     int64 -> float, two slices packed as re/im, row FFTs
 //   k_fft_cols_green   column FFT, x periodic Green's function, inverse
 //   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi
@@ -52,15 +57,8 @@ __device__ __forceinline__ void block_reduce(double (*sh)[BLOCK], double *v, con
     for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sc_fill(int64_t *grid, int32_t *cell, size_t ncell, int32_t *slc,
-                                                  int ns) {
-    const size_t stride = (size_t)gridDim.x * BLOCK;
-    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < ncell; i += stride) {
-        grid[i] = 0;
-        cell[i] = 0;
-    }
-    if (blockIdx.x == 0)
-        for (int s = threadIdx.x; s < ns; s += BLOCK) slc[s] = 0;
+__global__ __launch_bounds__(BLOCK) void k_sc_fill(int32_t *slc, int ns) {
+    for (int s = threadIdx.x; s < ns; s += BLOCK) slc[s] = 0;
 }
 
 // GetLocalCoordinates_2p5(_rk) + divide_arc2's scan (src/FFAG_SC_new.py:74-148, 534-566).
@@ -206,8 +204,18 @@ __global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_par
 }
 
 // Grid box from the moments (src/FFAG_Bunch.py:1259-1322).
-__global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res) {
+__global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res,
+                                                    const int32_t *slc, int32_t *slice_start, int32_t *cursor) {
     __shared__ double sh[4][BLOCK];
+    if (threadIdx.x == 0) {          // exclusive scan of the slice counts (ns <= 4096)
+        int32_t acc = 0;
+        for (int k = 0; k < sp.ns; ++k) {
+            slice_start[k] = acc;
+            cursor[k] = acc;
+            acc += slc[k];
+        }
+        slice_start[sp.ns] = acc;
+    }
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < NB; b += BLOCK)
         for (int k = 0; k < 4; ++k) v[k] += part[16 * b + 10 + k];
@@ -237,34 +245,94 @@ __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_
     res->qscale = bad ? 0.0 : (sp.q_macro / bl * sp.ns) * (1.0 / (dx * dz));
 }
 
-// Fixed-point CIC deposit (DistributeChargeSBS_Njit cell/weight rule,
-// src/FFAG_SC_new.py:244-312): a = rint(fx 2^bits), b = rint(fz 2^bits);
-// corners get (2^bits-a)(2^bits-b), a(2^bits-b), (2^bits-a)b, ab.
-__global__ __launch_bounds__(BLOCK) void k_sc_deposit(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
-                                                     const double *px, const int32_t *pslice,
-                                                     unsigned long long *grid, int32_t *cell) {
-    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-    if (i >= p.n) return;
-    const int s = pslice[i];
-    if (s < 0 || res->bad_grid != 0.0) return;
-    const int nx = sp.nx, nz = sp.nz;
+// CIC cell and fixed-point weights of one particle (DistributeChargeSBS_Njit
+// cell/weight rule, src/FFAG_SC_new.py:244-312): a = rint(fx 2^bits),
+// b = rint(fz 2^bits), packed as ix | iz << 10 | a << 20 | b << 41.
+__device__ __forceinline__ uint64_t cic_record(double x, double z, const ffa_sc_result *res, int nx, int nz, int bits) {
     const double inv_dx = 1.0 / res->dx, inv_dz = 1.0 / res->dz;
-    const double rx = (px[i] - res->xmin) * inv_dx, rz = (p.z[i] - res->zmin) * inv_dz;
+    const double rx = (x - res->xmin) * inv_dx, rz = (z - res->zmin) * inv_dz;
     const double crx = rx > -1e9 ? (rx < 1e9 ? rx : 1e9) : -1e9;
     const double crz = rz > -1e9 ? (rz < 1e9 ? rz : 1e9) : -1e9;
     int64_t ix = (int64_t)floor(crx), iz = (int64_t)floor(crz);
     double fx = rx - (double)ix, fz = rz - (double)iz;
     if (ix < 0) { ix = 0; fx = 0.0; } else if (ix >= nx - 1) { ix = nx - 2; fx = 1.0; }
     if (iz < 0) { iz = 0; fz = 0.0; } else if (iz >= nz - 1) { iz = nz - 2; fz = 1.0; }
-    const int64_t one = (int64_t)1 << sp.bits;
-    const double scale = (double)one;
-    const int64_t a = (int64_t)rint(fx * scale), b = (int64_t)rint(fz * scale);
-    const size_t base = (size_t)s * nz * nx + (size_t)iz * nx + (size_t)ix;
-    atomicAdd(&grid[base], (unsigned long long)((one - a) * (one - b)));
-    atomicAdd(&grid[base + 1], (unsigned long long)(a * (one - b)));
-    atomicAdd(&grid[base + nx], (unsigned long long)((one - a) * b));
-    atomicAdd(&grid[base + nx + 1], (unsigned long long)(a * b));
-    atomicAdd(&cell[base], 1);
+    const double scale = (double)((int64_t)1 << bits);
+    const uint64_t a = (uint64_t)rint(fx * scale), b = (uint64_t)rint(fz * scale);
+    return (uint64_t)ix | ((uint64_t)iz << 10) | (a << 20) | (b << 41);
+}
+
+// Bin the survivors into slice-sorted records.  Same block partition as
+// k_sc_stage2; pass 1 counts the block's particles per slice in LDS, one
+// global atomic per non-empty (block, slice) reserves a contiguous range,
+// pass 2 places each record with an LDS rank.
+__global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
+                                                 const double *px, const int32_t *pslice, int32_t *cursor,
+                                                 uint64_t *rec) {
+    __shared__ int32_t hist[MAX_NS_LDS];
+    if (res->bad_grid != 0.0) return;
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
+        const int s = pslice[i];
+        if (s >= 0) atomicAdd(&hist[s], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) {
+        const int c = hist[k];
+        hist[k] = c ? atomicAdd(&cursor[k], c) : 0;      // base of this block's range
+    }
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
+        const int s = pslice[i];
+        if (s < 0) continue;
+        const int pos = atomicAdd(&hist[s], 1);
+        rec[pos] = cic_record(px[i], p.z[i], res, sp.nx, sp.nz, sp.bits);
+    }
+}
+
+// One workgroup per (slice, band of bz rows): zero an LDS tile, add the
+// corners of every record of the slice that fall in the band, store the
+// band.  Dynamic LDS: bz * nx int64 sums, then bz * nx int32 base-cell counts.
+__global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, const ffa_sc_result *res,
+                                                           const int32_t *slice_start, const uint64_t *rec,
+                                                           unsigned long long *grid, int32_t *cell, int bz) {
+    extern __shared__ unsigned long long lds[];
+    const int nx = sp.nx, nz = sp.nz;
+    const int nbands = nz / bz;
+    const int s = blockIdx.x / nbands, band = blockIdx.x - s * nbands;
+    const int z0 = band * bz, z1 = z0 + bz;
+    const int ncell = bz * nx;
+    unsigned long long *acc = lds;
+    int32_t *cnt = reinterpret_cast<int32_t *>(lds + ncell);
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) { acc[k] = 0ull; cnt[k] = 0; }
+    __syncthreads();
+    if (res->bad_grid == 0.0) {
+        const uint64_t one = (uint64_t)1 << sp.bits;
+        const int r0 = slice_start[s], r1 = slice_start[s + 1];
+        for (int t = r0 + threadIdx.x; t < r1; t += BLOCK) {
+            const uint64_t w = rec[t];
+            const int ix = (int)(w & 1023u), iz = (int)((w >> 10) & 1023u);
+            const uint64_t a = (w >> 20) & 0x1FFFFFu, b = (w >> 41) & 0x1FFFFFu;
+            if (iz >= z0 && iz < z1) {
+                const int o = (iz - z0) * nx + ix;
+                atomicAdd(&acc[o], (one - a) * (one - b));
+                atomicAdd(&acc[o + 1], a * (one - b));
+                atomicAdd(&cnt[o], 1);
+            }
+            if (iz + 1 >= z0 && iz + 1 < z1) {
+                const int o = (iz + 1 - z0) * nx + ix;
+                atomicAdd(&acc[o], (one - a) * b);
+                atomicAdd(&acc[o + 1], a * b);
+            }
+        }
+    }
+    __syncthreads();
+    const size_t base = (size_t)s * nz * nx + (size_t)z0 * nx;
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) {
+        grid[base + k] = acc[k];
+        cell[base + k] = cnt[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -515,6 +583,9 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     L->slice_count = o; o = al(o + 4 * (size_t)ns);
     L->partials = o; o = al(o + sizeof(double) * 16 * NB);
     L->result = o; o = al(o + sizeof(ffa_sc_result));
+    L->records = o; o = al(o + 8 * (size_t)n);
+    L->slice_start = o; o = al(o + 4 * ((size_t)ns + 1));
+    L->cursor = o; o = al(o + 4 * (size_t)ns);
     L->total = o;
     return 0;
 }
@@ -536,15 +607,21 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(w + L->result);
     const double E0J = 938.2723e6 * 1.60217662e-19;   // src/FFAG_SC_new.py:115
     const size_t C = (size_t)sp->ns * sp->nz * sp->nx;
+    (void)C;
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_sc_fill, dim3(1024), dim3(BLOCK), 0, st, grid, cell, C, slc, sp->ns);
+    uint64_t *rec = reinterpret_cast<uint64_t *>(w + L->records);
+    int32_t *sstart = reinterpret_cast<int32_t *>(w + L->slice_start);
+    int32_t *cursor = reinterpret_cast<int32_t *>(w + L->cursor);
+    hipLaunchKernelGGL(k_sc_fill, dim3(1), dim3(BLOCK), 0, st, slc, sp->ns);
     hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, sp->c, E0J, part);
     hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
     hipLaunchKernelGGL(k_sc_stage2, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0, pslice, slc, part);
-    hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
-    if (p->n > 0)
-        hipLaunchKernelGGL(k_sc_deposit, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice,
-                           reinterpret_cast<unsigned long long *>(grid), cell);
+    hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart, cursor);
+    hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, cursor, rec);
+    const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
+    const size_t tile_lds = (size_t)bz * sp->nx * 12;
+    hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st, *sp,
+                       res, sstart, rec, reinterpret_cast<unsigned long long *>(grid), cell, bz);
     const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
