@@ -207,14 +207,28 @@ __global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_par
 __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res,
                                                     const int32_t *slc, int32_t *slice_start, int32_t *cursor) {
     __shared__ double sh[4][BLOCK];
-    if (threadIdx.x == 0) {          // exclusive scan of the slice counts (ns <= 4096)
-        int32_t acc = 0;
-        for (int k = 0; k < sp.ns; ++k) {
+    __shared__ int32_t tot[BLOCK];
+    {   // exclusive scan of the slice counts (ns <= 4096): each thread owns a
+        // contiguous run of ceil(ns / BLOCK) slices
+        const int per = (sp.ns + BLOCK - 1) / BLOCK;
+        const int k0 = threadIdx.x * per, k1 = min(k0 + per, sp.ns);
+        int32_t own = 0;
+        for (int k = k0; k < k1; ++k) own += slc[k];
+        tot[threadIdx.x] = own;
+        __syncthreads();
+        for (int w = 1; w < BLOCK; w <<= 1) {      // inclusive Hillis-Steele scan of the run totals
+            const int32_t add = threadIdx.x >= w ? tot[threadIdx.x - w] : 0;
+            __syncthreads();
+            tot[threadIdx.x] += add;
+            __syncthreads();
+        }
+        int32_t acc = tot[threadIdx.x] - own;
+        for (int k = k0; k < k1; ++k) {
             slice_start[k] = acc;
             cursor[k] = acc;
             acc += slc[k];
         }
-        slice_start[sp.ns] = acc;
+        if (threadIdx.x == BLOCK - 1) slice_start[sp.ns] = tot[BLOCK - 1];
     }
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < NB; b += BLOCK)
