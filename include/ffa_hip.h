@@ -214,6 +214,101 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
  * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);
 
+/* Static magnetic field at n points (device FP64 arrays r, fi, z; out: 3*n
+ * doubles, planes Bz, Bfi, Br) from the machine set by ffa_set_machine:
+ *   mode 0: GetBfields_Njit + Bmap_add_all_orders + harmonics
+ *           (src/FFAG_Field.py:872-884, 1333-1426; src/FFAG_MathTools.py:
+ *           1139-1188; src/FFAG_Field.py:1541-1587), the field the push uses;
+ *   mode 1: the Bmap_add_all_orders part alone (orders 1..max_order);
+ *   mode 2: that part in coefficient form for a trained range zt:
+ *           (sum_n Bz_n z^(2n-2) zt^2, sum_n Bfi_n z^(2n-2) zt^3,
+ *            sum_n Br_n z^(2n-2) zt^3), i.e. mode 1 divided by (z/zt)^2,
+ *           (z/zt)^3, (z/zt)^3.
+ * Used to label the surrogate's training samples and to judge its field. */
+int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *z, int64_t n, double *out,
+                    int32_t mode, double zt);
+
+/* ---------------------------------------------------------------------
+ * Field surrogate (no counterpart in the reference): a Fourier-feature MLP
+ * that stands in for the Taylor-map gather and z extrapolation
+ * Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) inside |z| <= zt.
+ *   a = (r - r_c) * inv_rh, u = frac(fi * inv_sector), c = z / zt
+ *   rev_f = m_f u + wa_f a + wc_f c^2 (f < 32, revolutions)
+ *   x0 = (sin 2 pi rev_0, cos 2 pi rev_0, sin 2 pi rev_1, ...)   (64)
+ *   x_{l+1} = silu(x_l W_l + b_l), l = 0..3                      (128)
+ *   o = x_4 W_4 + b_4                                             (3)
+ *   dBz = S_z c^2 o_0, dBfi = S_fi c^3 o_1, dBr = S_r c^3 o_2
+ * Parameters: FP32 master vector of FFA_MLP_NPARAM floats, W_l stored
+ * [in][out] row-major, in the order W0 b0 W1 b1 W2 b2 W3 b3 W4 b4.  The
+ * kernels run on a packed BF16 copy (FFA_MLP_PACKED_BYTES, zero-initialised
+ * by the caller once) with FP32 accumulation on the matrix cores.
+ * --------------------------------------------------------------------- */
+#define FFA_MLP_NFREQ 32
+#define FFA_MLP_NPARAM 58243
+#define FFA_MLP_PACKED_BYTES 236160
+
+typedef struct {
+    double r_c, inv_rh;        /* radial normalisation */
+    double inv_sector;         /* 1 / sector period [1/rad] */
+    double zt;                 /* trained |z| range [m] */
+    double out_scale[3];       /* S_z, S_fi, S_r [T] */
+    float m[FFA_MLP_NFREQ];    /* integer azimuthal harmonics */
+    float wa[FFA_MLP_NFREQ];   /* radial frequencies [rev] */
+    float wc[FFA_MLP_NFREQ];   /* z^2 frequencies [rev] */
+    int32_t fallback;          /* push: 1 = map gather for |z| > zt, 0 = order 0 only there */
+    int32_t pad;
+} ffa_mlp_desc;
+
+typedef struct {
+    float lr, beta1, beta2, eps;
+    int32_t step;              /* 1-based Adam step count */
+    int32_t pad;
+} ffa_adam;
+
+/* Workspace of one training/gradient call on a batch of n points. */
+typedef struct {
+    size_t total;
+    int64_t batch_padded;      /* n rounded up to FFA_MLP_KCHUNK */
+    size_t xfm;                /* bf16 activations, feature-major */
+    size_t dzfm;               /* bf16 backward signals, feature-major */
+    size_t zs;                 /* fp32 pre-activations, wave order */
+    size_t partials;           /* fp32 split-K weight-gradient partials */
+    size_t loss;               /* fp32 per-wave squared-error sums */
+} ffa_mlp_layout;
+
+#define FFA_MLP_KCHUNK 512
+
+int ffa_mlp_workspace_layout(int64_t n, ffa_mlp_layout *out);
+
+/* params (FP32 master) -> packed BF16 layouts. */
+int ffa_mlp_pack(void *ctx, const float *params, void *packed);
+
+/* Gradient of the mean squared error over the 3 n outputs against
+ * targets (3*n floats, planes o_0, o_1, o_2) at points (3*n floats,
+ * planes r, fi, z).  grad: FFA_MLP_NPARAM floats; loss: 1 float (device). */
+int ffa_mlp_grad(void *ctx, const ffa_mlp_desc *d, const void *packed, const float *pts, const float *targets,
+                 int64_t n, void *workspace, float *grad, float *loss);
+
+/* One Adam step (Kingma & Ba 2015, Algorithm 1) on params with moments m, v
+ * from grad, then repacks. */
+int ffa_mlp_adam(void *ctx, float *params, const float *grad, float *m, float *v, void *packed,
+                 const ffa_adam *hp);
+
+/* ffa_mlp_grad followed by ffa_mlp_adam without materialising grad. */
+int ffa_mlp_train_step(void *ctx, const ffa_mlp_desc *d, float *params, float *m, float *v, void *packed,
+                       const float *pts, const float *targets, int64_t n, void *workspace, const ffa_adam *hp,
+                       float *loss);
+
+/* Network outputs at n points (pts: 3*n floats r, fi, z): out 3*n floats,
+ * planes dBz, dBfi, dBr (mode 0) or raw outputs o (mode 1). */
+int ffa_mlp_eval(void *ctx, const ffa_mlp_desc *d, const void *packed, const float *pts, int64_t n, float *out,
+                 int32_t mode);
+
+/* Make ffa_push_boris_fused_f32 use the surrogate (packed != NULL) in place
+ * of the Taylor-map gather for particles with |z| <= zt, or stop using it
+ * (packed == NULL).  The packed buffer must stay alive while enabled. */
+int ffa_mlp_set(void *ctx, const ffa_mlp_desc *d, const void *packed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,7 +120,6 @@ class FFAG_RungeKutta:
         sc_step = int(LocalParameters.get("SC_step", 1))
         dumps = LocalParameters.get("step_dumps") or _Dumps()
         precision = LocalParameters.get("precision", BunchObj.precision)
-        verbose = LocalParameters.get("verbose", True)
         if enable_sc and sc_type != 1:
             raise TypeError("only SC_type=1 (2.5D slice-by-slice) is implemented on the GPU path")
         if solver == "rk4" and precision != "fp64":
@@ -130,9 +129,26 @@ class FFAG_RungeKutta:
         max_steps = int(stop["max_stepsN"])
         max_fi = stop["max_turn"] * np.pi * 2
         max_time = stop["max_time"]
+        surrogate = LocalParameters.get("surrogate")
+        if surrogate is not None and not (precision == "fp32" and solver == "boris"):
+            raise ValueError("the field surrogate drives the FP32 Boris tier only")
         self._setup(BunchObj, GlobalParameters, precision)
         ctx = self.ctx
         lib = ctx.lib
+        if surrogate is not None:
+            surrogate.enable(ctx)
+        try:
+            return self._loop(t_start, BunchObj, GlobalParameters, LocalParameters, solver, precision, dumps,
+                              enable_sc, sc_step, step_t, max_steps, max_fi, max_time)
+        finally:
+            if surrogate is not None:
+                surrogate.disable(ctx)
+
+    def _loop(self, t_start, BunchObj, GlobalParameters, LocalParameters, solver, precision, dumps, enable_sc,
+              sc_step, step_t, max_steps, max_fi, max_time):
+        ctx = self.ctx
+        lib = ctx.lib
+        verbose = LocalParameters.get("verbose", True)
         has_dumps = len(getattr(dumps, "dumps", [1])) > 0
         t_pre = t_start
         steps_i = int(LocalParameters.get("start_step", 0))

@@ -83,6 +83,29 @@ class FfaScLayout(ctypes.Structure):
         "slice_count", "partials", "result", "records", "slice_start", "cursor")]
 
 
+MLP_NFREQ = 32
+MLP_NPARAM = 58243
+MLP_PACKED_BYTES = 236160
+MLP_KCHUNK = 512
+
+
+class FfaMlpDesc(ctypes.Structure):
+    _fields_ = [("r_c", _d), ("inv_rh", _d), ("inv_sector", _d), ("zt", _d), ("out_scale", _d * 3),
+                ("m", ctypes.c_float * MLP_NFREQ), ("wa", ctypes.c_float * MLP_NFREQ),
+                ("wc", ctypes.c_float * MLP_NFREQ), ("fallback", _i32), ("pad", _i32)]
+
+
+class FfaAdam(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("step", _i32), ("pad", _i32)]
+
+
+class FfaMlpLayout(ctypes.Structure):
+    _fields_ = [("total", ctypes.c_size_t), ("batch_padded", _i64), ("xfm", ctypes.c_size_t),
+                ("dzfm", ctypes.c_size_t), ("zs", ctypes.c_size_t), ("partials", ctypes.c_size_t),
+                ("loss", ctypes.c_size_t)]
+
+
 class FfaError(RuntimeError):
     pass
 
@@ -93,7 +116,9 @@ _lib = None
 EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ffa_set_machine",
             "ffa_inject", "ffa_push_boris_f64", "ffa_push_rk4_f64", "ffa_push_f64_multi",
             "ffa_push_boris_fused_f32",
-            "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi")
+            "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
+            "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
+            "ffa_mlp_eval", "ffa_mlp_set")
 
 
 def load():
@@ -121,6 +146,15 @@ def load():
     lib.ffa_sc_workspace_layout.argtypes = [_i64, _i32, _i32, _i32, ctypes.POINTER(FfaScLayout)]
     lib.ffa_sc_solve.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams), _vp]
     lib.ffa_reduce_phi.argtypes = [_vp, ctypes.POINTER(FfaParticles), _vp]
+    lib.ffa_bfield_eval.argtypes = [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _d]
+    lib.ffa_mlp_workspace_layout.argtypes = [_i64, ctypes.POINTER(FfaMlpLayout)]
+    lib.ffa_mlp_pack.argtypes = [_vp, _vp, _vp]
+    lib.ffa_mlp_grad.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp, _vp, _vp, _i64, _vp, _vp, _vp]
+    lib.ffa_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FfaAdam)]
+    lib.ffa_mlp_train_step.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+                                       ctypes.POINTER(FfaAdam), _vp]
+    lib.ffa_mlp_eval.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp, _vp, _i64, _vp, _i32]
+    lib.ffa_mlp_set.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp]
     for name in EXPORTED:
         if name != "ffa_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -161,4 +195,13 @@ def sc_layout(n: int, ns: int, nz: int, nx: int) -> FfaScLayout:
     rc = lib.ffa_sc_workspace_layout(_i64(n), _i32(ns), _i32(nz), _i32(nx), ctypes.byref(L))
     if rc != 0:
         raise FfaError(f"ffa_sc_workspace_layout failed ({rc}): {lib.ffa_last_error(None).decode()}")
+    return L
+
+
+def mlp_layout(n: int) -> FfaMlpLayout:
+    lib = load()
+    L = FfaMlpLayout()
+    rc = lib.ffa_mlp_workspace_layout(_i64(n), ctypes.byref(L))
+    if rc != 0:
+        raise FfaError(f"ffa_mlp_workspace_layout failed ({rc}): {lib.ffa_last_error(None).decode()}")
     return L

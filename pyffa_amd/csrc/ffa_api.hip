@@ -238,6 +238,113 @@ int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {
     return hip_check(c, launch_reduce_phi(c, p, out_dev), "ffa_reduce_phi");
 }
 
+int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *z, int64_t n, double *out,
+                    int32_t mode, double zt) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(c, FFA_ERR_INVALID, "ctx is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (n < 0) return fail(c, FFA_ERR_INVALID, "negative point count");
+    if (mode < 0 || mode > 2) return fail(c, FFA_ERR_INVALID, "mode must be 0, 1 or 2");
+    if (mode == 2 && !(zt > 0.0)) return fail(c, FFA_ERR_INVALID, "mode 2 needs zt > 0");
+    if (n > 0 && (!r || !fi || !z || !out)) return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "ffa_bfield_eval needs the FP64 Taylor map");
+    return hip_check(c, launch_bfield_eval(c, r, fi, z, n, out, mode, zt), "ffa_bfield_eval");
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_mlp_desc(ffa_ctx_s *c, const ffa_mlp_desc *d) {
+    if (!d) return fail(c, FFA_ERR_INVALID, "surrogate descriptor is NULL");
+    if (!(d->zt > 0.0) || !(d->inv_rh > 0.0) || !(d->inv_sector > 0.0))
+        return fail(c, FFA_ERR_INVALID, "surrogate descriptor needs zt, inv_rh, inv_sector > 0");
+    for (int k = 0; k < FFA_MLP_NFREQ; ++k)
+        if (d->m[k] != std::floor(d->m[k]))
+            return fail(c, FFA_ERR_INVALID, "azimuthal harmonic m[%d] = %g is not an integer", k, (double)d->m[k]);
+    return FFA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffa_mlp_workspace_layout(int64_t n, ffa_mlp_layout *out) {
+    if (!out) return fail(nullptr, FFA_ERR_INVALID, "out is NULL");
+    if (n <= 0 || n > (int64_t)1 << 26) return fail(nullptr, FFA_ERR_INVALID, "batch must be in [1, 2^26]");
+    mlp::workspace_layout(n, *out);
+    return FFA_OK;
+}
+
+int ffa_mlp_pack(void *ctx, const float *params, void *packed) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !params || !packed) return fail(c, FFA_ERR_INVALID, "ctx, params or packed is NULL");
+    return hip_check(c, launch_mlp_pack(c, params, packed), "ffa_mlp_pack");
+}
+
+int ffa_mlp_grad(void *ctx, const ffa_mlp_desc *d, const void *packed, const float *pts, const float *targets,
+                 int64_t n, void *workspace, float *grad, float *loss) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !packed || !pts || !targets || !workspace || !grad)
+        return fail(c, FFA_ERR_INVALID, "a pointer argument is NULL");
+    if (int rc = check_mlp_desc(c, d)) return rc;
+    if (n <= 0 || n > (int64_t)1 << 26) return fail(c, FFA_ERR_INVALID, "batch must be in [1, 2^26]");
+    return hip_check(c,
+                     launch_mlp_grad(c, d, packed, pts, targets, n, workspace, nullptr, grad, nullptr, nullptr,
+                                     const_cast<void *>(packed), nullptr, loss),
+                     "ffa_mlp_grad");
+}
+
+int ffa_mlp_adam(void *ctx, float *params, const float *grad, float *m, float *v, void *packed, const ffa_adam *hp) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !params || !grad || !m || !v || !packed || !hp) return fail(c, FFA_ERR_INVALID, "a pointer is NULL");
+    if (hp->step < 1) return fail(c, FFA_ERR_INVALID, "Adam step count starts at 1");
+    return hip_check(c, launch_mlp_adam(c, params, grad, m, v, packed, hp), "ffa_mlp_adam");
+}
+
+int ffa_mlp_train_step(void *ctx, const ffa_mlp_desc *d, float *params, float *m, float *v, void *packed,
+                       const float *pts, const float *targets, int64_t n, void *workspace, const ffa_adam *hp,
+                       float *loss) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !params || !m || !v || !packed || !pts || !targets || !workspace || !hp)
+        return fail(c, FFA_ERR_INVALID, "a pointer argument is NULL");
+    if (int rc = check_mlp_desc(c, d)) return rc;
+    if (n <= 0 || n > (int64_t)1 << 26) return fail(c, FFA_ERR_INVALID, "batch must be in [1, 2^26]");
+    if (hp->step < 1) return fail(c, FFA_ERR_INVALID, "Adam step count starts at 1");
+    if (c->mlp_packed == packed)
+        return fail(c, FFA_ERR_STATE, "these weights are in use by the push (ffa_mlp_set); train a copy");
+    return hip_check(c,
+                     launch_mlp_grad(c, d, packed, pts, targets, n, workspace, params, nullptr, m, v, packed, hp,
+                                     loss),
+                     "ffa_mlp_train_step");
+}
+
+int ffa_mlp_eval(void *ctx, const ffa_mlp_desc *d, const void *packed, const float *pts, int64_t n, float *out,
+                 int32_t mode) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !packed) return fail(c, FFA_ERR_INVALID, "ctx or packed is NULL");
+    if (int rc = check_mlp_desc(c, d)) return rc;
+    if (n < 0) return fail(c, FFA_ERR_INVALID, "negative point count");
+    if (n > 0 && (!pts || !out)) return fail(c, FFA_ERR_INVALID, "pts or out is NULL");
+    if (mode != 0 && mode != 1) return fail(c, FFA_ERR_INVALID, "mode must be 0 or 1");
+    return hip_check(c, launch_mlp_eval(c, d, packed, pts, n, out, mode), "ffa_mlp_eval");
+}
+
+int ffa_mlp_set(void *ctx, const ffa_mlp_desc *d, const void *packed) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(c, FFA_ERR_INVALID, "ctx is NULL");
+    if (!packed) {
+        c->mlp_packed = nullptr;
+        return FFA_OK;
+    }
+    if (int rc = check_mlp_desc(c, d)) return rc;
+    hipError_t e = mlp_upload_desc(c, d);
+    if (e != hipSuccess) return hip_check(c, e, "ffa_mlp_set");
+    c->mlp_packed = packed;
+    return FFA_OK;
+}
+
 }  // extern "C"
 
 void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {

@@ -16,7 +16,7 @@
 // kernel carries u = gamma v, so the Boris step needs two reciprocal square
 // roots and one division (the reference recomputes gamma from v each step;
 // the two agree to FP64 rounding).
-#include "ffa_internal.h"
+#include "ffa_mlp.h"
 
 namespace {
 
@@ -26,7 +26,8 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 
 __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float z,
                                            float xf, float yf, float v0, float rf_red, float &Bz, float &Br,
-                                           float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf) {
+                                           float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
+                                           bool taylor = true) {
     // fast_mod on the branch-cut azimuth (equal to the unwrapped one mod the sector)
     const float fim = th - k->P * floorf(th * k->invP);
     float spiral = 0.0f, spiral_p = 0.0f;
@@ -55,7 +56,7 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
     Bz = B0r * F;
     Bfi = B0r * Fp * inv_r * z;
     Br = (B0r_p * F - B0r * Fp * spiral_p) * z;
-    if (k->no >= 1) {
+    if (k->no >= 1 && taylor) {
         float fx = (r - k->r_min) * k->inv_rstep;
         float fy = (fim - k->fi_min) * k->inv_fistep;
         fx = fminf(fmaxf(fx, -1.0f), (float)k->NX);
@@ -142,27 +143,44 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
     }
 }
 
-template <typename ET>
-__global__ __launch_bounds__(BLOCK) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
+// SUR: the field surrogate (weights staged in LDS, one 512-thread workgroup
+// per CU) replaces the Taylor-map gather for particles with |z| <= zt; every
+// lane runs the step loop to the end so the wave-wide matrix-core work sees
+// all 64 lanes.
+constexpr int SUR_BLOCK = 512;
+
+template <typename ET, bool SUR>
+__global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
                                                       const ffa_fconst *__restrict__ kc,
                                                       const ffa_step *__restrict__ steps,
-                                                      const int64_t *__restrict__ inj, int nsteps) {
-    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-    if (i >= p.n) return;
+                                                      const int64_t *__restrict__ inj, int nsteps,
+                                                      const char *__restrict__ sur_w,
+                                                      const mlp::fdesc *__restrict__ md) {
+    const int64_t i = blockIdx.x * (int64_t)(SUR ? SUR_BLOCK : BLOCK) + threadIdx.x;
+    __shared__ __attribute__((aligned(16))) char lds[SUR ? mlp::FWD_BYTES : 16];
+    if constexpr (SUR) {
+        for (int k = threadIdx.x; k < mlp::FWD_BYTES / 16; k += SUR_BLOCK)
+            reinterpret_cast<uint4 *>(lds)[k] = reinterpret_cast<const uint4 *>(sur_w)[k];
+        __syncthreads();
+    } else {
+        if (i >= p.n) return;
+    }
+    const bool inb = i < p.n;
+    const int64_t ii = inb ? i : 0;    // lanes past the end load particle 0 and never store
     const ffa_machine &m = *mp;
     const int64_t n = p.n;
-    uint8_t fl = p.flags[i];
-    double rf = p.rf_phase[i];
+    uint8_t fl = inb ? p.flags[ii] : (uint8_t)0;
+    double rf = p.rf_phase[ii];
     const ET *ep = reinterpret_cast<const ET *>(p.esc_pre);
     const ET *epp = reinterpret_cast<const ET *>(p.esc_prepre);
-    const float e0r = (float)ep[i], e0z = (float)ep[n + i], e0f = (float)ep[2 * n + i];
-    const float d_r = e0r - (float)epp[i], d_z = e0z - (float)epp[n + i], d_f = e0f - (float)epp[2 * n + i];
-    const double r0 = p.r[i], fi0 = p.fi[i];
+    const float e0r = (float)ep[ii], e0z = (float)ep[n + ii], e0f = (float)ep[2 * n + ii];
+    const float d_r = e0r - (float)epp[ii], d_z = e0z - (float)epp[n + ii], d_f = e0f - (float)epp[2 * n + ii];
+    const double r0 = p.r[ii], fi0 = p.fi[ii];
     double s0, c0;
     sincos(fi0, &s0, &c0);
-    double x = r0 * c0, y = r0 * s0, z = p.z[i];
-    const double vr0 = p.vr[i], rfd = r0 * p.fidot[i];
-    const double vx0 = vr0 * c0 - rfd * s0, vy0 = vr0 * s0 + rfd * c0, vz0 = p.vz[i];
+    double x = r0 * c0, y = r0 * s0, z = p.z[ii];
+    const double vr0 = p.vr[ii], rfd = r0 * p.fidot[ii];
+    const double vx0 = vr0 * c0 - rfd * s0, vy0 = vr0 * s0 + rfd * c0, vz0 = p.vz[ii];
     const double q = m.q, mm = m.m0, cl = m.c;
     const double inv_c2 = 1.0 / (cl * cl);
     // u = gamma v
@@ -179,6 +197,23 @@ __global__ __launch_bounds__(BLOCK) void k_boris_fused(ffa_particles p, const ff
         const ffa_step &s = steps[kstep];
         if (i >= inj[2 * kstep] && i < inj[2 * kstep + 1]) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
         const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
+        bool use_sur = false;
+        float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
+        if constexpr (SUR) {
+            const float xq = (float)x, yq = (float)y, zq = (float)z;
+            float a, u, c;
+            mlp::inputs(md, sqrtf(xq * xq + yq * yq), atan2f(yq, xq), zq, a, u, c);
+            use_sur = live && fabsf(c) <= 1.0f;
+            if (__ballot(use_sur) != 0) {          // wave-uniform: all 64 lanes evaluate
+                float o[3];
+                mlp::forward_wave(reinterpret_cast<const __bf16 *>(lds),
+                                  reinterpret_cast<const float *>(lds + mlp::FWD_BF16 * 2), md, a, u, c * c, o);
+                const float c2 = c * c, c3 = c2 * c;
+                sBz = md->S[0] * c2 * o[0];
+                sBfi = md->S[1] * c3 * o[1];
+                sBr = md->S[2] * c3 * o[2];
+            }
+        }
         if (!live) { rf += s.dRF; continue; }
         moved = true;
         float esr = e0r, esz = e0z, esf = e0f;
@@ -191,7 +226,13 @@ __global__ __launch_bounds__(BLOCK) void k_boris_fused(ffa_particles p, const ff
         const float th = atan2f(yf, xf);
         const double rfr = rf - FFA_TWO_PI * rint(rf * (1.0 / FFA_TWO_PI));
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
-        fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, (float)rfr, Bz, Br, Bfi, Er, Efi, Ez, srf);
+        fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, (float)rfr, Bz, Br, Bfi, Er, Efi, Ez, srf,
+                   SUR ? (!use_sur && md->fallback != 0) : true);
+        if constexpr (SUR) {
+            Bz += sBz;
+            Bfi += sBfi;
+            Br += sBr;
+        }
         const float ir = rcpf(rr);
         const float cf = xf * ir, sf = yf * ir;
         const float ert = Er * srf + esr, eft = Efi * srf + esf, ezt = Ez * srf + esz;
@@ -223,6 +264,7 @@ __global__ __launch_bounds__(BLOCK) void k_boris_fused(ffa_particles p, const ff
         }
         rf += s.dRF;
     }
+    if (!inb) return;
     p.rf_phase[i] = rf;
     if (fl != p.flags[i]) p.flags[i] = fl;
     if (!moved) return;
@@ -244,12 +286,24 @@ hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t 
     if (p->n == 0 || nsteps == 0) return hipSuccess;
     const ffa_step *steps = reinterpret_cast<const ffa_step *>(c->dev + FFA_CTX_STEPS_OFF);
     const int64_t *inj = reinterpret_cast<const int64_t *>(c->dev + FFA_CTX_INJ_OFF);
+    const mlp::fdesc *md = reinterpret_cast<const mlp::fdesc *>(c->dev + FFA_CTX_MLP_OFF);
+    const char *w = static_cast<const char *>(c->mlp_packed);
+    if (c->mlp_packed != nullptr) {
+        const unsigned g = (unsigned)((p->n + SUR_BLOCK - 1) / SUR_BLOCK);
+        if (p->esc_f32)
+            hipLaunchKernelGGL((k_boris_fused<float, true>), dim3(g), dim3(SUR_BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+        else
+            hipLaunchKernelGGL((k_boris_fused<double, true>), dim3(g), dim3(SUR_BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+        return hipGetLastError();
+    }
     const unsigned g = (unsigned)((p->n + BLOCK - 1) / BLOCK);
     if (p->esc_f32)
-        hipLaunchKernelGGL(k_boris_fused<float>, dim3(g), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
-                           ctx_fconst_dev(c), steps, inj, nsteps);
+        hipLaunchKernelGGL((k_boris_fused<float, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
+                           ctx_fconst_dev(c), steps, inj, nsteps, w, md);
     else
-        hipLaunchKernelGGL(k_boris_fused<double>, dim3(g), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
-                           ctx_fconst_dev(c), steps, inj, nsteps);
+        hipLaunchKernelGGL((k_boris_fused<double, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p,
+                           ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
     return hipGetLastError();
 }

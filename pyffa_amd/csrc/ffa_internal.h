@@ -44,6 +44,7 @@ struct ffa_ctx_s {
     //   [..., +64K)    reduction partials / scalars
     char *dev = nullptr;
     size_t dev_bytes = 0;
+    const void *mlp_packed = nullptr;   // surrogate weights used by the FP32 push (null: off)
 };
 
 static constexpr size_t FFA_CTX_MACHINE_OFF = 0;
@@ -53,7 +54,9 @@ static constexpr size_t FFA_CTX_INJ_OFF = FFA_CTX_STEPS_OFF + FFA_CTX_STEPS_BYTE
 static constexpr size_t FFA_CTX_INJ_BYTES = 64 * 1024;
 static constexpr size_t FFA_CTX_RED_OFF = FFA_CTX_INJ_OFF + FFA_CTX_INJ_BYTES;
 static constexpr size_t FFA_CTX_RED_BYTES = 64 * 1024;
-static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_RED_OFF + FFA_CTX_RED_BYTES;
+static constexpr size_t FFA_CTX_MLP_OFF = FFA_CTX_RED_OFF + FFA_CTX_RED_BYTES;   // surrogate descriptor
+static constexpr size_t FFA_CTX_MLP_BYTES = 4096;
+static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_MLP_OFF + FFA_CTX_MLP_BYTES;
 static constexpr int FFA_MAX_FUSED = (int)(FFA_CTX_STEPS_BYTES / sizeof(ffa_step));
 
 static constexpr size_t FFA_CTX_FCONST_OFF = 6144;   // after the machine (< 6 KB)
@@ -72,5 +75,19 @@ hipError_t launch_boris_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step
 hipError_t launch_rk4_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s);
 hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps);
 hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_dev);
+hipError_t launch_bfield_eval(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
+                              double *out, int mode, double zt);
+hipError_t launch_mlp_pack(ffa_ctx_s *c, const float *params, void *packed);
+hipError_t launch_mlp_grad(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *packed, const float *pts,
+                           const float *tgt, int64_t n, void *ws, float *params, float *grad, float *mom, float *vel,
+                           void *packed_out, const ffa_adam *hp, float *loss);
+hipError_t launch_mlp_adam(ffa_ctx_s *c, float *params, const float *grad, float *mom, float *vel, void *packed,
+                           const ffa_adam *hp);
+hipError_t launch_mlp_eval(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *packed, const float *pts, int64_t n,
+                           float *out, int mode);
+hipError_t mlp_upload_desc(ffa_ctx_s *c, const ffa_mlp_desc *d);   // -> FFA_CTX_MLP_OFF (synchronous)
+namespace mlp {
+size_t workspace_layout(int64_t n, ffa_mlp_layout &L);
+}
 hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp,
                            void *ws, const ffa_sc_layout *L);

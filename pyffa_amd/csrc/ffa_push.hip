@@ -57,9 +57,11 @@ __device__ __forceinline__ void order0_f64(const ffa_machine &m, double ri, doub
     Br = (B0r_p * F - B0r * Fp * spiral_p) * zi;
 }
 
-// Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) on the interleaved map.
-__device__ __forceinline__ void taylor_add_f64(const ffa_machine &m, double x, double y, double z,
-                                               double &oBz, double &oBfi, double &oBr) {
+// Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) on the interleaved map,
+// with the z powers of order 1 passed in: (z2, z^2, z^3) for the field,
+// (z2, zt^2, zt^3) for the coefficient form of ffa_bfield_eval mode 2.
+__device__ __forceinline__ void taylor_add_pow_f64(const ffa_machine &m, double x, double y, double z2, double ze0,
+                                                   double zo0, double &oBz, double &oBfi, double &oBr) {
     const int NX = m.r_size, NY = m.fi_size, no = m.max_order;
     double fx = (x - m.r_min) / m.r_step;
     double fy = (y - m.fi_min) / m.fi_step;
@@ -77,8 +79,7 @@ __device__ __forceinline__ void taylor_add_f64(const ffa_machine &m, double x, d
     const int64_t stride_j = 3 * (int64_t)no;
     const double *a0 = T + (i * NY + j) * stride_j;          // (i, j), (i, j+1) contiguous
     const double *a1 = T + ((i + 1) * NY + j) * stride_j;    // (i+1, j), (i+1, j+1)
-    const double z2 = z * z;
-    double ze = z2, zo = ze * z;
+    double ze = ze0, zo = zo0;
     for (int n = 0; n < no; ++n) {
         const double *p00 = a0 + 3 * n, *p01 = a0 + stride_j + 3 * n;
         const double *p10 = a1 + 3 * n, *p11 = a1 + stride_j + 3 * n;
@@ -90,6 +91,12 @@ __device__ __forceinline__ void taylor_add_f64(const ffa_machine &m, double x, d
         oBr += br * zo;
         if (n != no - 1) { ze *= z2; zo *= z2; }
     }
+}
+
+__device__ __forceinline__ void taylor_add_f64(const ffa_machine &m, double x, double y, double z,
+                                               double &oBz, double &oBfi, double &oBr) {
+    const double z2 = z * z;
+    taylor_add_pow_f64(m, x, y, z2, z2, z2 * z, oBz, oBfi, oBr);
 }
 
 // bharmonics_kernel_1d (src/FFAG_Field.py:1541-1587): raw azimuth.
@@ -338,6 +345,32 @@ __global__ __launch_bounds__(BLOCK) void k_phi_final(const double *__restrict__ 
     if (threadIdx.x == 0) { out[0] = ss[0]; out[1] = sc[0]; }
 }
 
+// ffa_bfield_eval: static field at points (order 0 + Taylor + harmonics, the
+// Taylor part, or its coefficient form).
+__global__ __launch_bounds__(BLOCK) void k_bfield_eval(const ffa_machine *__restrict__ mp, const double *__restrict__ r,
+                                                       const double *__restrict__ fi, const double *__restrict__ z,
+                                                       int64_t n, double *__restrict__ out, int mode, double zt) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const ffa_machine &m = *mp;
+    const double P = FFA_PI * 2.0 / (double)m.nsectors;
+    const double ri = r[i], fii = fi[i], zi = z[i];
+    const double fim = fii - P * floor(fii / P);
+    double Bz = 0.0, Br = 0.0, Bfi = 0.0;
+    if (mode == 0) {
+        order0_f64(m, ri, fim, zi, Bz, Br, Bfi);
+        if (m.max_order >= 1) taylor_add_f64(m, ri, fim, zi, Bz, Bfi, Br);
+        if (m.n_harm > 0) harmonics_f64(m, ri, fii, Bz);
+    } else if (m.max_order >= 1) {
+        const double z2 = zi * zi;
+        if (mode == 1) taylor_add_pow_f64(m, ri, fim, z2, z2, z2 * zi, Bz, Bfi, Br);
+        else taylor_add_pow_f64(m, ri, fim, z2, zt * zt, zt * zt * zt, Bz, Bfi, Br);
+    }
+    out[i] = Bz;
+    out[n + i] = Bfi;
+    out[2 * n + i] = Br;
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
@@ -364,5 +397,13 @@ hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_d
     double *part = reinterpret_cast<double *>(c->dev + FFA_CTX_RED_OFF);
     hipLaunchKernelGGL(k_phi_partials, dim3(RED_BLOCKS), dim3(BLOCK), 0, c->stream, p->fi, p->n, part);
     hipLaunchKernelGGL(k_phi_final, dim3(1), dim3(BLOCK), 0, c->stream, part, out_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_bfield_eval(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
+                              double *out, int mode, double zt) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bfield_eval, dim3(grid_for(n)), dim3(BLOCK), 0, c->stream, ctx_machine_dev(c), r, fi, z, n,
+                       out, mode, zt);
     return hipGetLastError();
 }

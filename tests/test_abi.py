@@ -30,6 +30,28 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FfaParticles) == 10 * 8 + 8 + 8
     assert ctypes.sizeof(_lib.FfaScLayout) == 16 * 8
     assert ctypes.sizeof(_lib.FfaScResult) == 32 * 8
+    assert ctypes.sizeof(_lib.FfaMlpDesc) == 7 * 8 + 3 * 32 * 4 + 8
+    assert ctypes.sizeof(_lib.FfaAdam) == 6 * 4
+    assert ctypes.sizeof(_lib.FfaMlpLayout) == 7 * 8
+
+
+def test_header_constants_match_binding():
+    src = open(os.path.join(ROOT, "include", "ffa_hip.h")).read()
+    for name, val in (("FFA_MLP_NFREQ", _lib.MLP_NFREQ), ("FFA_MLP_NPARAM", _lib.MLP_NPARAM),
+                      ("FFA_MLP_PACKED_BYTES", _lib.MLP_PACKED_BYTES), ("FFA_MLP_KCHUNK", _lib.MLP_KCHUNK)):
+        m = re.search(r"#define\s+%s\s+(\d+)" % name, src)
+        assert m and int(m.group(1)) == val, name
+
+
+def test_mlp_workspace_layout():
+    L = _lib.mlp_layout(1000)
+    assert L.batch_padded == 1024
+    offs = [L.xfm, L.dzfm, L.zs, L.partials, L.loss]
+    assert all(o % 256 == 0 for o in offs) and offs == sorted(offs) and L.total > offs[-1]
+    assert L.dzfm - L.xfm >= (64 + 4 * 128) * 1024 * 2
+    assert L.loss - L.partials >= 2 * _lib.MLP_NPARAM * 4
+    lib = _lib.load()
+    assert lib.ffa_mlp_workspace_layout(0, ctypes.byref(L)) == -1
 
 
 def test_workspace_layout_is_aligned_and_ordered():
