@@ -1,16 +1,23 @@
 """Throughput benchmark of the FFAG tracking hot path on one MI355X.
 
-Workload: 12-sector scaling FFAG (r0 = 8.5 m, k = 7.6, 300 MeV protons,
-order-2 Taylor maps on an r 8.0-9.3 m x one-sector grid), 1M macro-particles
-(seeded Gaussian bunch, uniform azimuth), 2.5D space charge on a
-(ns, nx, nz) = (256, 128, 128) grid at +-4 sigma every 20 push steps, two RF
-gaps 180 degrees apart at 100 kV with a linear frequency ramp, Boris pusher,
-dt = 0.05 ns.  One bench step = 20 push steps + one SC solve.
+Workload (headline): 12-sector scaling FFAG (r0 = 8.5 m, k = 7.6, 300 MeV
+protons, order-2 Taylor maps on an r 8.0-9.3 m x one-sector grid), 1M
+macro-particles (seeded Gaussian bunch, uniform azimuth), 2.5D space charge
+on a (ns, nx, nz) = (256, 128, 128) grid at +-4 sigma every 20 push steps,
+two RF gaps 180 degrees apart at 100 kV with a linear frequency ramp, Boris
+pusher, dt = 0.05 ns.  One bench step = 20 push steps + one SC solve.
 
 The headline value is the FP32-field tier (FP64 state, fused steps); the FP64
 map-gather path is timed on the same workload for the speed-up, and the
 accuracy of the FP32 tier is reported against it (centroid, rms emittance).
-The CPU baseline is the OpenMP FP64 oracle on the box's host cores.
+
+Further legs (each a sub-object of the JSON line):
+* ``surrogate`` -- the field network trained on the GPU on 4M samples of the
+  order-4 Taylor-map field, driving the FP32 tier on the same workload with
+  order-4 maps as fallback; beside it the FP32 and FP64 order-4 map tiers;
+  training time, field error, matrix-core throughput of the network;
+* ``scale_8m`` -- 8M particles with a (512, 256, 256) grid, FP32 vs FP64;
+* ``cpu_baseline`` -- the OpenMP FP64 oracle on the host cores.
 """
 import argparse
 import json
@@ -23,17 +30,22 @@ sys.path.insert(0, ROOT)
 
 N_PART = 1_000_000
 GRID = (256, 128, 128, 4.0)      # (ns, nx, nz, sigma_ratio)
+GRID_8M = (512, 256, 256, 4.0)
 SC_STEP = 20
 DT = 0.05e-9
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+# dense matrix-core FLOPs of one network evaluation: 2 * (64*128 + 3*128*128 + 128*3)
+MLP_FLOP = 2 * (64 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
-def build_inputs(n, tmpdir):
-    import numpy as np
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def machines(max_order, tmpdir):
     from tests import cases
-    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=100e3), aperture=cases.APERTURE, tmpdir=tmpdir)
-    M = cases.bunch("stable", n=n, seed=0)
-    return mc, gp, M
+    return cases.machines("stable", max_order=max_order, rf=dict(v0=100e3), aperture=cases.APERTURE, tmpdir=tmpdir)
 
 
 def snapshot(B):
@@ -55,19 +67,32 @@ def moments(S, alive):
                 alive=int(alive.sum()))
 
 
-def run_tier(precision, M, gp, warmup, steps):
+def accuracy(Sa, Sb, steps):
+    """Moments of tier a against tier b over the particles alive in both."""
+    both = ((Sa["flags"] & 2) != 0) & ((Sb["flags"] & 2) != 0)
+    ma, mb = moments(Sa, both), moments(Sb, both)
+    return {"centroid_r_abs_m": abs(ma["r_mean"] - mb["r_mean"]), "centroid_z_abs_m": abs(ma["z_mean"] - mb["z_mean"]),
+            "emit_r_rel": abs(ma["emit_r"] / mb["emit_r"] - 1.0), "emit_z_rel": abs(ma["emit_z"] / mb["emit_z"] - 1.0),
+            "alive_a": int(((Sa["flags"] & 2) != 0).sum()), "alive_b": int(((Sb["flags"] & 2) != 0).sum()),
+            "compared": ma["alive"], "after_steps": steps}
+
+
+def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None):
     import torch
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
     from tests import cases
-    B = FFAG_Bunch(M, marcosize=1e13 / M.shape[0], sc_grid_size=GRID, BunchType="Boris", precision=precision)
+    B = FFAG_Bunch(M, marcosize=1e13 / M.shape[0], sc_grid_size=grid, BunchType="Boris", precision=precision)
     cases.set_bunch_seo(B)
     rk = FFAG_RungeKutta()
 
     def lp(first, last):
-        return dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0),
-                    time_step=DT, step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP,
-                    precision=precision, verbose=False, start_step=first * SC_STEP)
+        d = dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0),
+                 time_step=DT, step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP,
+                 precision=precision, verbose=False, start_step=first * SC_STEP)
+        if surrogate is not None:
+            d["surrogate"] = surrogate
+        return d
     t = 0.0
     if warmup:
         rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(0, warmup))
@@ -82,13 +107,14 @@ def run_tier(precision, M, gp, warmup, steps):
 
 def kernel_time_ms(rk, B, gp, precision, reps=10):
     """Average duration of the dominant kernel launch (fused 20-step push for
-    the FP32 tier) and of one SC solve, with HIP events on the stream the
-    kernels run on."""
+    the FP32 tier, one step for FP64) and of one SC solve, with HIP events
+    recorded on the stream the kernels run on."""
     import torch
     from pyffa_amd import _lib
     import numpy as np
     ctx = rk.ctx
     stream = torch.cuda.current_stream()
+    assert ctx is not None
     steps = (_lib.FfaStep * SC_STEP)()
     inj = (_lib._i64 * (2 * SC_STEP))()
     t = rk.last_run["t"]
@@ -121,7 +147,6 @@ def kernel_time_ms(rk, B, gp, precision, reps=10):
 
 
 def cpu_baseline(M, mc, n_sample, macro_steps):
-    import numpy as np
     from oracle import oracle as O
     from tests import cases
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
@@ -137,6 +162,65 @@ def cpu_baseline(M, mc, n_sample, macro_steps):
                        f"{macro_steps} SC solves (FP64 OpenMP oracle), {el:.1f} s")
 
 
+def surrogate_leg(M, tmp, warmup, steps, train_iters):
+    """Train the network on the GPU, then run the FP32 tier with it against
+    the FP32 and FP64 order-4 map tiers on the headline workload."""
+    import torch
+    from pyffa_amd.FFAG_Surrogate import FFAG_FieldSurrogate
+    _, gp4 = machines(4, tmp)
+    sur = FFAG_FieldSurrogate(gp4, zt=0.01)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sur.build_dataset(4_000_000, seed=1)
+    hist = sur.train(iters=train_iters, batch=8192, lr=2e-3, seed=2, log_every=max(1, train_iters // 4))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    err = sur.field_error(n=1_000_000, seed=9)
+    log(f"surrogate trained in {t_train:.1f} s, field error {err}")
+    # matrix-core throughput of the network alone (4M points, HIP events)
+    pts = torch.stack(sur.sample_points(1 << 22, 3)).float().contiguous()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    sur.eval(pts)
+    ev[0].record()
+    for _ in range(10):
+        sur.eval(pts)
+    ev[1].record()
+    ev[1].synchronize()
+    t_eval = ev[0].elapsed_time(ev[1]) / 10 * 1e-3
+    tflops = pts.shape[1] * MLP_FLOP / t_eval / 1e12
+    out = {"train_seconds": t_train, "train_samples": 4_000_000, "train_iters": train_iters, "batch": 8192,
+           "loss_history": hist, "field_error": err, "zt_m": sur.zt,
+           "mfma": {"kernel": "k_mlp_eval (4M points)", "ms": t_eval * 1e3, "achieved_tflops": tflops,
+                    "peak_tflops": BF16_PEAK_TFLOPS, "frac": tflops / BF16_PEAK_TFLOPS}}
+    ps = M.shape[0] * steps * SC_STEP
+    B, rk, el_sur, S_sur = run_tier("fp32", M, gp4, warmup, steps, surrogate=sur)
+    del B, rk
+    B, rk, el_map32, S_map32 = run_tier("fp32", M, gp4, warmup, steps)
+    del B, rk
+    B, rk, el_map64, S_map64 = run_tier("fp64", M, gp4, warmup, steps)
+    del B, rk
+    after = (warmup + steps) * SC_STEP
+    out.update({"value": ps / el_sur, "map_order4_fp32_value": ps / el_map32, "map_order4_fp64_value": ps / el_map64,
+                "speedup_vs_fp64_order4": el_map64 / el_sur,
+                "accuracy_vs_fp64_order4": accuracy(S_sur, S_map64, after),
+                "fp32_map_accuracy_vs_fp64_order4": accuracy(S_map32, S_map64, after)})
+    log(f"surrogate tier {ps / el_sur:.3e}, fp32 order-4 {ps / el_map32:.3e}, fp64 order-4 {ps / el_map64:.3e}")
+    return out
+
+
+def scale_leg(tmp, n, steps):
+    """FP32 tier vs FP64 path at 8M particles on the (512, 256, 256) grid."""
+    from tests import cases
+    _, gp = machines(2, tmp)
+    M = cases.bunch("stable", n=n, seed=0)
+    ps = n * steps * SC_STEP
+    _, _, el32, S32 = run_tier("fp32", M, gp, 1, steps, grid=GRID_8M)
+    _, _, el64, S64 = run_tier("fp64", M, gp, 1, steps, grid=GRID_8M)
+    return {"particles": n, "sc_grid": list(GRID_8M[:3]), "steps": steps, "value_fp32": ps / el32,
+            "value_fp64": ps / el64, "speedup_fp32_vs_fp64": el64 / el32,
+            "accuracy_fp32_vs_fp64": accuracy(S32, S64, (1 + steps) * SC_STEP)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,18 +231,35 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=50)
+    ap.add_argument("--train-iters", type=int, default=6000)
+    ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--scale-particles", type=int, default=8_000_000)
+    ap.add_argument("--scale-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
     import tempfile
     import torch
+    if args.copy_calibration:
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 256 MiB read + 256 MiB written")
     tmp = tempfile.mkdtemp()
-    mc, gp, M = build_inputs(args.particles, tmp)
-    print(f"[bench] inputs ready: {M.shape[0]} particles", file=sys.stderr, flush=True)
+    from tests import cases
+    mc, gp = machines(2, tmp)
+    M = cases.bunch("stable", n=args.particles, seed=0)
+    log(f"inputs ready: {M.shape[0]} particles")
     B32, rk32, el32, S32 = run_tier("fp32", M, gp, args.warmup, args.steps)
     k32 = kernel_time_ms(rk32, B32, gp, "fp32")
-    print(f"[bench] fp32 tier {el32:.3f} s, kernels {k32}", file=sys.stderr, flush=True)
+    log(f"fp32 tier {el32:.3f} s, kernels {k32}")
+    del B32, rk32
     n_alive = M.shape[0]
     ps = n_alive * args.steps * SC_STEP
     out = {
@@ -166,39 +267,39 @@ def main():
         "value": ps / el32, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el32 / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 state / f32 field", "data": "synthetic",
-        "config": {"workload": "cfg1: 1M protons (1e13 total), 12-sector scaling FFAG r0=8.5 m k=7.6 "
-                               "(D=-1.4 vertically stable variant), order-2 Taylor maps, aperture r 8.0-9.3 m |z|<5 cm, 2.5D SC (ns,nx,nz)=(256,128,128) every 20 steps, 2 RF gaps "
-                               "100 kV, Boris dt=0.05 ns; FP32-field tier",
+        "config": {"workload": "1M protons (1e13 total), 12-sector scaling FFAG r0=8.5 m k=7.6 "
+                               "(D=-1.4 vertically stable variant), order-2 Taylor maps, aperture r 8.0-9.3 m "
+                               "|z|<5 cm, 2.5D SC (ns,nx,nz)=(256,128,128) every 20 steps, 2 RF gaps 100 kV, "
+                               "Boris dt=0.05 ns; FP32-field tier",
                    "particles": n_alive, "sc_grid": list(GRID[:3]), "sc_step": SC_STEP},
     }
     if not args.no_fp64:
         B64, rk64, el64, S64 = run_tier("fp64", M, gp, args.warmup, args.steps)
-        both = ((S32["flags"] & 2) != 0) & ((S64["flags"] & 2) != 0)
-        m32, m64 = moments(S32, both), moments(S64, both)
         k64 = kernel_time_ms(rk64, B64, gp, "fp64")
-        print(f"[bench] fp64 path {el64:.3f} s, kernels {k64}", file=sys.stderr, flush=True)
+        del B64, rk64
+        log(f"fp64 path {el64:.3f} s, kernels {k64}")
         out["fp64_value"] = ps / el64
         out["speedup_fp32_vs_fp64"] = el64 / el32
-        out["accuracy_fp32_vs_fp64"] = {
-            "centroid_r_abs_m": abs(m32["r_mean"] - m64["r_mean"]),
-            "centroid_z_abs_m": abs(m32["z_mean"] - m64["z_mean"]),
-            "emit_r_rel": abs(m32["emit_r"] / m64["emit_r"] - 1.0),
-            "emit_z_rel": abs(m32["emit_z"] / m64["emit_z"] - 1.0),
-            "alive_fp32": int(((S32["flags"] & 2) != 0).sum()), "alive_fp64": int(((S64["flags"] & 2) != 0).sum()),
-            "compared": m32["alive"], "after_steps": (args.warmup + args.steps) * SC_STEP}
+        out["accuracy_fp32_vs_fp64"] = accuracy(S32, S64, (args.warmup + args.steps) * SC_STEP)
         out["fp64_kernel_ms"] = k64
-    # roofline of the dominant kernel of the headline tier: the fused 20-step push.
-    # Algorithmic HBM bytes per particle per launch: FP64 state read + write
-    # (7 x 8 x 2), flag byte read + write, FP32 SC endpoints Pre/PrePre (6 x 4).
+    # roofline of the dominant kernel of the headline tier: the fused 20-step
+    # push.  Algorithmic HBM bytes per particle per launch: FP64 state read +
+    # write (7 x 8 x 2), flag byte read + write, FP32 SC endpoints Pre/PrePre
+    # (6 x 4).
     bytes_per_particle = 7 * 8 * 2 + 2 + 6 * 4
-    traffic = n_alive * bytes_per_particle
-    ach = traffic / (k32["push"] * 1e-3) / 1e9
+    algo = n_alive * bytes_per_particle
+    ach = algo / (k32["push"] * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_boris_fused<float> (20 steps)",
                        "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"]}
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(M, tmp, 2, args.surrogate_steps, args.train_iters)
+    if not args.no_scale:
+        out["scale_8m"] = scale_leg(tmp, args.scale_particles, args.scale_steps)
+        log(f"8M leg {out['scale_8m']}")
     if not args.no_cpu:
         cb = cpu_baseline(M, mc, min(args.cpu_particles, M.shape[0]), args.cpu_steps)
-        print(f"[bench] cpu baseline {cb}", file=sys.stderr, flush=True)
+        log(f"cpu baseline {cb}")
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

@@ -237,7 +237,10 @@ int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *
  *   x0 = (sin 2 pi rev_0, cos 2 pi rev_0, sin 2 pi rev_1, ...)   (64)
  *   x_{l+1} = silu(x_l W_l + b_l), l = 0..3                      (128)
  *   o = x_4 W_4 + b_4                                             (3)
- *   dBz = S_z c^2 o_0, dBfi = S_fi c^3 o_1, dBr = S_r c^3 o_2
+ *   dBz = S_z c^2 g(o_0), dBfi = S_fi c^3 g(o_1), dBr = S_r c^3 g(o_2)
+ *   g(o) = o, or delta sinh(o) when out_delta = delta > 0 (the network then
+ *   learns asinh(coefficient / (S delta)): relative accuracy where the
+ *   Taylor coefficients are large, absolute accuracy where they are small)
  * Parameters: FP32 master vector of FFA_MLP_NPARAM floats, W_l stored
  * [in][out] row-major, in the order W0 b0 W1 b1 W2 b2 W3 b3 W4 b4.  The
  * kernels run on a packed BF16 copy (FFA_MLP_PACKED_BYTES, zero-initialised
@@ -252,6 +255,7 @@ typedef struct {
     double inv_sector;         /* 1 / sector period [1/rad] */
     double zt;                 /* trained |z| range [m] */
     double out_scale[3];       /* S_z, S_fi, S_r [T] */
+    double out_delta;          /* 0: linear outputs; > 0: delta sinh(o) */
     float m[FFA_MLP_NFREQ];    /* integer azimuthal harmonics */
     float wa[FFA_MLP_NFREQ];   /* radial frequencies [rev] */
     float wc[FFA_MLP_NFREQ];   /* z^2 frequencies [rev] */

@@ -18,8 +18,9 @@ Network (same definition as the HIP kernels):
   feature ``2f = sin(2 pi rev_f)``, ``2f+1 = cos(2 pi rev_f)``; the integer
   ``m_f`` make the features periodic over the sector;
 * four hidden layers of 128 SiLU units, then 3 linear outputs ``o``;
-* field ``dBz = S_z c^2 o_0``, ``dBfi = S_fi c^3 o_1``, ``dBr = S_r c^3 o_2``,
-  so the z parity of the Taylor series holds exactly.
+* field ``dBz = S_z c^2 g(o_0)``, ``dBfi = S_fi c^3 g(o_1)``,
+  ``dBr = S_r c^3 g(o_2)`` with ``g(o) = o`` or ``delta sinh(o)``, so the z
+  parity of the Taylor series holds exactly.
 
 ``forward``/``backward`` can round to bfloat16 exactly where the GPU does
 (weights, features, hidden activations and the backward signals that feed the
@@ -136,7 +137,9 @@ def adam_step(p, g, mom, vel, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     p[:] = p - lr * mh / (np.sqrt(vh) + eps)
 
 
-def field(o, c, scale):
-    """Network outputs to the Taylor-correction field (dBz, dBfi, dBr)."""
+def field(o, c, scale, delta=0.0):
+    """Network outputs to the Taylor-correction field (dBz, dBfi, dBr); with
+    delta > 0 the outputs are first mapped through delta * sinh."""
+    g = delta * np.sinh(o) if delta > 0 else o
     c2 = c * c
-    return np.stack([scale[0] * c2 * o[:, 0], scale[1] * c2 * c * o[:, 1], scale[2] * c2 * c * o[:, 2]], 1)
+    return np.stack([scale[0] * c2 * g[:, 0], scale[1] * c2 * c * g[:, 1], scale[2] * c2 * c * g[:, 2]], 1)

@@ -11,9 +11,13 @@ form of the map gather at random points of one sector, evaluated in FP64 by
 analytic; outside ``|z| <= zt`` the push falls back to the map gather.
 
 The network's z dependence is built in: it predicts ``o(r, theta, (z/zt)^2)``
-and the field is ``dBz = S_z c^2 o_0``, ``dBfi = S_fi c^3 o_1``,
-``dBr = S_r c^3 o_2`` with ``c = z / zt``, which keeps the parity of the
+and the field is ``dBz = S_z c^2 g(o_0)``, ``dBfi = S_fi c^3 g(o_1)``,
+``dBr = S_r c^3 g(o_2)`` with ``c = z / zt``, which keeps the parity of the
 Taylor series and the vanishing of the correction on the mid-plane exact.
+``g(o) = o`` by default; with ``delta > 0``, ``g(o) = delta sinh(o)`` and the
+network is trained on ``asinh(coefficient / (S delta))``, which trades
+accuracy at the fringes for accuracy in the magnet bodies.  On the 12-sector
+test lattice the linear output tracks the map method best over 2000 steps.
 
 Typical use::
 
@@ -68,7 +72,7 @@ def init_params(seed=0):
 
 
 class FFAG_FieldSurrogate:
-    def __init__(self, GlobalParameters, zt=0.01, seed=0, device="cuda", fallback=True, ctx=None):
+    def __init__(self, GlobalParameters, zt=0.01, seed=0, device="cuda", fallback=True, ctx=None, delta=0.0):
         import torch
         self.torch = torch
         self.gp = GlobalParameters
@@ -82,6 +86,7 @@ class FFAG_FieldSurrogate:
         self.sector = 2.0 * np.pi / float(bm.Nsectors)
         self.freq = default_frequencies()
         self.scale = np.ones(3)
+        self.delta = float(delta)
         self.fallback = bool(fallback)
         self.ctx = ctx or _lib.Context(self.device.index or 0, torch.cuda.current_stream(self.device).cuda_stream)
         self._machine_tay = None
@@ -105,6 +110,7 @@ class FFAG_FieldSurrogate:
         d.zt = self.zt
         for k in range(3):
             d.out_scale[k] = float(self.scale[k])
+        d.out_delta = self.delta
         m, wa, wc = self.freq
         for k in range(_lib.MLP_NFREQ):
             d.m[k], d.wa[k], d.wc[k] = float(m[k]), float(wa[k]), float(wc[k])
@@ -164,7 +170,10 @@ class FFAG_FieldSurrogate:
         self.scale[self.scale == 0] = 1.0
         s = torch.tensor(self.scale, dtype=torch.float64, device=self.device)[:, None]
         self.pts = torch.stack([r, fi, z]).float().contiguous()
-        self.tgt = (coef / s).float().contiguous()
+        t = coef / s
+        if self.delta > 0:
+            t = torch.asinh(t / self.delta)
+        self.tgt = t.float().contiguous()
         self.n_data = n
         return self
 
@@ -272,13 +281,14 @@ class FFAG_FieldSurrogate:
         ctx.check(ctx.lib.ffa_mlp_set(ctx.h, None, None), "ffa_mlp_set")
 
     def state_dict(self):
-        return dict(params=self.params.cpu().numpy(), scale=np.asarray(self.scale), zt=self.zt,
+        return dict(params=self.params.cpu().numpy(), scale=np.asarray(self.scale), zt=self.zt, delta=self.delta,
                     freq=[np.asarray(f) for f in self.freq], step=self.step)
 
     def load_state_dict(self, sd):
         self.params.copy_(self.torch.tensor(np.asarray(sd["params"], np.float32), device=self.device))
         self.scale = np.asarray(sd["scale"], np.float64)
         self.zt = float(sd["zt"])
+        self.delta = float(sd.get("delta", 0.0))
         self.freq = tuple(np.asarray(f, np.float32) for f in sd["freq"])
         self.step = int(sd.get("step", 0))
         self._pack()

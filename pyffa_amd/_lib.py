@@ -90,7 +90,7 @@ MLP_KCHUNK = 512
 
 
 class FfaMlpDesc(ctypes.Structure):
-    _fields_ = [("r_c", _d), ("inv_rh", _d), ("inv_sector", _d), ("zt", _d), ("out_scale", _d * 3),
+    _fields_ = [("r_c", _d), ("inv_rh", _d), ("inv_sector", _d), ("zt", _d), ("out_scale", _d * 3), ("out_delta", _d),
                 ("m", ctypes.c_float * MLP_NFREQ), ("wa", ctypes.c_float * MLP_NFREQ),
                 ("wc", ctypes.c_float * MLP_NFREQ), ("fallback", _i32), ("pad", _i32)]
 

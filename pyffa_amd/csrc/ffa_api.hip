@@ -260,6 +260,7 @@ int check_mlp_desc(ffa_ctx_s *c, const ffa_mlp_desc *d) {
     if (!d) return fail(c, FFA_ERR_INVALID, "surrogate descriptor is NULL");
     if (!(d->zt > 0.0) || !(d->inv_rh > 0.0) || !(d->inv_sector > 0.0))
         return fail(c, FFA_ERR_INVALID, "surrogate descriptor needs zt, inv_rh, inv_sector > 0");
+    if (!(d->out_delta >= 0.0)) return fail(c, FFA_ERR_INVALID, "out_delta must be >= 0");
     for (int k = 0; k < FFA_MLP_NFREQ; ++k)
         if (d->m[k] != std::floor(d->m[k]))
             return fail(c, FFA_ERR_INVALID, "azimuthal harmonic m[%d] = %g is not an integer", k, (double)d->m[k]);

@@ -208,10 +208,8 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
                 float o[3];
                 mlp::forward_wave(reinterpret_cast<const __bf16 *>(lds),
                                   reinterpret_cast<const float *>(lds + mlp::FWD_BF16 * 2), md, a, u, c * c, o);
-                const float c2 = c * c, c3 = c2 * c;
-                sBz = md->S[0] * c2 * o[0];
-                sBfi = md->S[1] * c3 * o[1];
-                sBr = md->S[2] * c3 * o[2];
+                // lanes outside |z| <= zt keep the map gather (fallback) and no network term
+                if (use_sur) mlp::field_from_outputs(md, c, o, sBz, sBfi, sBr);
             }
         }
         if (!live) { rf += s.dRF; continue; }

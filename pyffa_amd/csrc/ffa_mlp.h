@@ -59,6 +59,7 @@ __host__ __device__ constexpr int perm16(int k) { return (k & ~12) | ((k & 4) <<
 struct fdesc {
     float r_c, inv_rh, inv_sector, zt, inv_zt;
     float S[3];
+    float delta;                   // output map: 0 -> o, > 0 -> delta sinh(o)
     float m[NFREQ], wa[NFREQ], wc[NFREQ];
     int32_t fallback, pad[3];
 };
@@ -68,6 +69,21 @@ __device__ __forceinline__ float silu(float x) { return x * __builtin_amdgcn_rcp
 __device__ __forceinline__ float dsilu(float x) {
     const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-x));
     return s * (1.0f + x * (1.0f - s));
+}
+
+// Taylor-part field of a point from the network outputs o at c = z / zt
+__device__ __forceinline__ void field_from_outputs(const fdesc *d, float c, const float o[3], float &Bz, float &Bfi,
+                                                   float &Br) {
+    float g0 = o[0], g1 = o[1], g2 = o[2];
+    if (d->delta > 0.0f) {
+        g0 = 0.5f * d->delta * (__expf(g0) - __expf(-g0));
+        g1 = 0.5f * d->delta * (__expf(g1) - __expf(-g1));
+        g2 = 0.5f * d->delta * (__expf(g2) - __expf(-g2));
+    }
+    const float c2 = c * c, c3 = c2 * c;
+    Bz = d->S[0] * c2 * g0;
+    Bfi = d->S[1] * c3 * g1;
+    Br = d->S[2] * c3 * g2;
 }
 
 // network inputs of a point

@@ -296,10 +296,11 @@ __global__ __launch_bounds__(EVAL_BLOCK) void k_mlp_eval(const char *__restrict_
         forward_wave(W, b, &d, a, u, c * c, o);
         if (valid) {
             if (mode == 0) {
-                const float c2 = c * c, c3 = c2 * c;
-                out[p] = d.S[0] * c2 * o[0];
-                out[n + p] = d.S[1] * c3 * o[1];
-                out[2 * n + p] = d.S[2] * c3 * o[2];
+                float bz, bfi, br;
+                field_from_outputs(&d, c, o, bz, bfi, br);
+                out[p] = bz;
+                out[n + p] = bfi;
+                out[2 * n + p] = br;
             } else {
                 out[p] = o[0];
                 out[n + p] = o[1];
@@ -321,6 +322,7 @@ fdesc fdesc_from(const ffa_mlp_desc &d) {
     f.zt = (float)d.zt;
     f.inv_zt = (float)(1.0 / d.zt);
     for (int k = 0; k < 3; ++k) f.S[k] = (float)d.out_scale[k];
+    f.delta = (float)d.out_delta;
     for (int k = 0; k < NFREQ; ++k) {
         f.m[k] = d.m[k];
         f.wa[k] = d.wa[k];

@@ -30,7 +30,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FfaParticles) == 10 * 8 + 8 + 8
     assert ctypes.sizeof(_lib.FfaScLayout) == 16 * 8
     assert ctypes.sizeof(_lib.FfaScResult) == 32 * 8
-    assert ctypes.sizeof(_lib.FfaMlpDesc) == 7 * 8 + 3 * 32 * 4 + 8
+    assert ctypes.sizeof(_lib.FfaMlpDesc) == 8 * 8 + 3 * 32 * 4 + 8
     assert ctypes.sizeof(_lib.FfaAdam) == 6 * 4
     assert ctypes.sizeof(_lib.FfaMlpLayout) == 7 * 8
 
