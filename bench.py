@@ -39,6 +39,21 @@ BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP = 2 * (64 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_cfg1_1M_fused20.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch (read + written) of a kernel of this workload,
+    from the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled by a
+    256 MiB copy) summarised in profiles/; None when no summary is present."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            k = json.load(f)["kernels"][kernel]
+        return k["read_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -290,7 +305,8 @@ def main():
     algo = n_alive * bytes_per_particle
     ach = algo / (k32["push"] * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_boris_fused<float> (20 steps)",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_boris_fused<float, false>"),
+                       "algorithmic_bytes": algo, "kernel": "k_boris_fused<float> (20 steps)",
                        "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"]}
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(M, tmp, 2, args.surrogate_steps, args.train_iters)
