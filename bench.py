@@ -64,7 +64,7 @@ def machines(max_order, tmpdir):
 
 
 def snapshot(B):
-    return {k: getattr(B, k).clone() for k in ("r", "vr", "z", "vz", "fidot", "flags")}
+    return {k: B.ordered(k) for k in ("r", "vr", "z", "vz", "fidot", "flags")}
 
 
 def moments(S, alive):

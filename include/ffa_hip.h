@@ -214,6 +214,19 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
  * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);
 
+/* Particle order (no counterpart in the reference, which keeps the input
+ * order): a stable sort of particles [0, count) by azimuth bin
+ * floor((fi mod 2 pi) * nbins / 2 pi) (particles not injected-and-alive
+ * last), written into a second set of arrays dst of the same size and
+ * element types, particles [count, n) copied unchanged.  pid_src / pid_dst
+ * (int64, n) carry each particle's input index so the caller can restore the
+ * input order.  Neighbouring threads then read neighbouring Taylor-map cells
+ * in the push and neighbouring slices in the SC gather.  nbins in
+ * [1, 65535]; workspace: ffa_sort_workspace_size(count) bytes. */
+int ffa_sort_workspace_size(int64_t count, size_t *bytes);
+int ffa_sort_particles(void *ctx, const ffa_particles *src, const ffa_particles *dst, int64_t count, int32_t nbins,
+                       const int64_t *pid_src, int64_t *pid_dst, void *workspace);
+
 /* Static magnetic field at n points (device FP64 arrays r, fi, z; out: 3*n
  * doubles, planes Bz, Bfi, Br) from the machine set by ffa_set_machine:
  *   mode 0: GetBfields_Njit + Bmap_add_all_orders + harmonics

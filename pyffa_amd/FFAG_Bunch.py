@@ -14,6 +14,8 @@ Column layout of the mirror (``FFAG_ManageBunchAttribute``, src/FFAG_Bunch.py:
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -120,18 +122,64 @@ class FFAG_Bunch:
         esc = t.from_numpy(np.ascontiguousarray(M[:, 10:13].T)).to(dev, edt).contiguous()
         self.esc_pre = esc
         self.esc_prepre = esc.clone()
+        # input index of the particle held at each position (the FP32 tier
+        # reorders particles by azimuth; the host mirror restores the order)
+        self.pid = t.arange(n, dtype=t.int64, device=dev)
+        self._alt = None
+        self._sort_ws = None
         self._meta = M[:, [6, 13, 14, 15]].copy()
         self._host_cache = None
 
-    def particles(self) -> "_lib.FfaParticles":
+    _STATE = ("r", "vr", "z", "vz", "fi", "fidot", "rf_phase", "flags", "esc_pre", "esc_prepre", "pid")
+
+    def _particles_of(self, a) -> "_lib.FfaParticles":
         p = _lib.FfaParticles()
-        p.r, p.vr, p.z, p.vz = self.r.data_ptr(), self.vr.data_ptr(), self.z.data_ptr(), self.vz.data_ptr()
-        p.fi, p.fidot, p.rf_phase = self.fi.data_ptr(), self.fidot.data_ptr(), self.rf_phase.data_ptr()
-        p.flags = self.flags.data_ptr()
-        p.esc_pre, p.esc_prepre = self.esc_pre.data_ptr(), self.esc_prepre.data_ptr()
+        p.r, p.vr, p.z, p.vz = a["r"].data_ptr(), a["vr"].data_ptr(), a["z"].data_ptr(), a["vz"].data_ptr()
+        p.fi, p.fidot, p.rf_phase = a["fi"].data_ptr(), a["fidot"].data_ptr(), a["rf_phase"].data_ptr()
+        p.flags = a["flags"].data_ptr()
+        p.esc_pre, p.esc_prepre = a["esc_pre"].data_ptr(), a["esc_prepre"].data_ptr()
         p.n = self.TotalParticleNum_Local
         p.esc_f32 = 1 if self.precision == "fp32" else 0
         return p
+
+    def particles(self) -> "_lib.FfaParticles":
+        return self._particles_of({k: getattr(self, k) for k in self._STATE})
+
+    def sort_by_azimuth(self, ctx=None, nbins=65535):
+        """Reorder the injected particles by azimuth bin (stable; see
+        ffa_sort_particles).  The host mirror and ``ordered()`` keep
+        presenting particles in the input order."""
+        count = self._uninject_head
+        if count < 2:
+            return
+        if ctx is None:
+            ctx = self._own_ctx()
+        t = self.torch
+        if self._alt is None:
+            self._alt = {k: t.empty_like(getattr(self, k)) for k in self._STATE}
+        if self._sort_ws is None or self._sort_ws[0] < count:
+            nbytes = ctypes.c_size_t()
+            ctx.check(ctx.lib.ffa_sort_workspace_size(count, ctypes.byref(nbytes)), "ffa_sort_workspace_size")
+            self._sort_ws = (count, t.empty(max(nbytes.value, 1), dtype=t.uint8, device=self.device))
+        alt = self._alt
+        ctx.check(ctx.lib.ffa_sort_particles(ctx.h, self.particles(), self._particles_of(alt), count, int(nbins),
+                                             _lib._vp(self.pid.data_ptr()), _lib._vp(alt["pid"].data_ptr()),
+                                             _lib._vp(self._sort_ws[1].data_ptr())), "ffa_sort_particles")
+        cur = {k: getattr(self, k) for k in self._STATE}
+        for k in self._STATE:
+            setattr(self, k, alt[k])
+        self._alt = cur
+        self._host_cache = None
+
+    def ordered(self, name):
+        """Device copy of a state array in the input particle order."""
+        a = getattr(self, name)
+        out = self.torch.empty_like(a)
+        if a.dim() == 2:
+            out[:, self.pid] = a
+        else:
+            out[self.pid] = a
+        return out
 
     def invalidate_host(self):
         self._host_cache = None
@@ -144,15 +192,14 @@ class FFAG_Bunch:
             t = self.torch
             n = self.TotalParticleNum_Local
             M = np.zeros((n, 16))
-            for j, a in ((0, self.r), (1, self.vr), (2, self.z), (3, self.vz), (4, self.fi), (5, self.fidot),
-                         (9, self.rf_phase)):
-                M[:, j] = a.cpu().numpy()
-            fl = self.flags.cpu().numpy()
+            for j, name in ((0, "r"), (1, "vr"), (2, "z"), (3, "vz"), (4, "fi"), (5, "fidot"), (9, "rf_phase")):
+                M[:, j] = self.ordered(name).cpu().numpy()
+            fl = self.ordered("flags").cpu().numpy()
             M[:, 7] = (fl & _lib.F_INJ) != 0
             M[:, 8] = (fl & _lib.F_ALIVE) != 0
             M[:, [6, 13, 14, 15]] = self._meta
-            pre = self.esc_pre.to(t.float64).cpu().numpy()
-            pp = self.esc_prepre.to(t.float64).cpu().numpy()
+            pre = self.ordered("esc_pre").to(t.float64).cpu().numpy()
+            pp = self.ordered("esc_prepre").to(t.float64).cpu().numpy()
             both = ((fl & _lib.F_SCPRE) != 0) & ((fl & _lib.F_SCPREPRE) != 0)
             esc = pre.copy()
             esc[:, both] = pre[:, both] + self.last_sc_s * (pre[:, both] - pp[:, both])
@@ -169,7 +216,7 @@ class FFAG_Bunch:
 
     @property
     def Step_SurviveFlag(self):
-        return (self.flags.cpu().numpy() & _lib.F_ALIVE) != 0
+        return (self.ordered("flags").cpu().numpy() & _lib.F_ALIVE) != 0
 
     # ----------------------------------------------------------- bookkeeping
     def injection_range(self, t_threshold, t_step):

@@ -91,11 +91,12 @@ class FFAG_RungeKutta:
 
     def _progress(self, BunchObj, steps_i, t_post):
         fl = BunchObj.flags
-        idx = (fl & _lib.F_ALIVE).nonzero()
+        idx = (fl & _lib.F_ALIVE).nonzero()[:, 0]
         BunchObj.UpdateParticleNumGlobal()
         clock = time.strftime("%m-%d %H:%M:%S", time.localtime())
         if idx.numel() > 0:
-            i = int(idx[0, 0])
+            # the first alive particle in input order (the FP32 tier reorders)
+            i = int(idx[int(BunchObj.pid[idx].argmin())])
             r = float(BunchObj.r[i])
             fi = float(BunchObj.fi[i])
             print(f"current step={steps_i:.0f}, r ~= {r:.2f} m, turn ~= {fi / np.pi / 2:.2f}, "
@@ -150,6 +151,9 @@ class FFAG_RungeKutta:
         lib = ctx.lib
         verbose = LocalParameters.get("verbose", True)
         has_dumps = len(getattr(dumps, "dumps", [1])) > 0
+        # FP32 tier: reorder particles by azimuth every sort_every steps (map
+        # and SC-grid locality); the FP64 path keeps the input order
+        sort_every = int(LocalParameters.get("sort_every", 1000 if precision == "fp32" else 0))
         t_pre = t_start
         steps_i = int(LocalParameters.get("start_step", 0))
         max_block = int(LocalParameters.get("max_block", _MAX_BLOCK))
@@ -163,6 +167,8 @@ class FFAG_RungeKutta:
             nb = 1000 - steps_i % 1000
             if enable_sc:
                 nb = min(nb, sc_step - steps_i % sc_step)
+            if sort_every > 0:
+                nb = min(nb, sort_every - steps_i % sort_every)
             nb = min(nb, max_steps - steps_i, max(1, min(max_block, _MAX_BLOCK)))
             if has_dumps:
                 nb = 1
@@ -170,6 +176,8 @@ class FFAG_RungeKutta:
             if e0 > b0:
                 ctx.check(lib.ffa_inject(ctx.h, BunchObj.particles(), b0, e0), "ffa_inject")
                 BunchObj._account_injection(b0, e0)
+            if sort_every > 0 and steps_i % sort_every == 0:
+                BunchObj.sort_by_azimuth(ctx)
             if enable_sc and steps_i % sc_step == 0:
                 s_last = ((steps_i - 1) % sc_step) / sc_step if steps_i - 1 >= sc_step else 0.0
                 BunchObj.sc_solve_device(ctx, first=(steps_i == 0), s_last=s_last)

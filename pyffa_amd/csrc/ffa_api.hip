@@ -238,6 +238,31 @@ int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {
     return hip_check(c, launch_reduce_phi(c, p, out_dev), "ffa_reduce_phi");
 }
 
+int ffa_sort_workspace_size(int64_t count, size_t *bytes) {
+    if (!bytes) return fail(nullptr, FFA_ERR_INVALID, "bytes is NULL");
+    if (count < 0 || count > 0xFFFFFFFFLL) return fail(nullptr, FFA_ERR_INVALID, "count must be in [0, 2^32)");
+    *bytes = sort_workspace_bytes(count);
+    return FFA_OK;
+}
+
+int ffa_sort_particles(void *ctx, const ffa_particles *src, const ffa_particles *dst, int64_t count, int32_t nbins,
+                       const int64_t *pid_src, int64_t *pid_dst, void *workspace) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !workspace || !pid_src || !pid_dst) return fail(c, FFA_ERR_INVALID, "a pointer argument is NULL");
+    if (int rc = check_particles(c, src)) return rc;
+    if (int rc = check_particles(c, dst)) return rc;
+    if (!dst || dst->n != src->n || dst->esc_f32 != src->esc_f32)
+        return fail(c, FFA_ERR_INVALID, "dst must have the size and element types of src");
+    if (count < 0 || count > src->n || count > 0xFFFFFFFFLL)
+        return fail(c, FFA_ERR_INVALID, "count %lld outside [0, n]", (long long)count);
+    if (nbins < 1 || nbins > 65535) return fail(c, FFA_ERR_INVALID, "nbins must be in [1, 65535]");
+    if (src->r == dst->r || pid_src == pid_dst) return fail(c, FFA_ERR_INVALID, "src and dst must not alias");
+    const uint32_t *order = nullptr;
+    hipError_t e = launch_sort_order(c, src, count, nbins, workspace, &order);
+    if (e == hipSuccess) e = launch_gather_particles(c, src, dst, order, count, pid_src, pid_dst);
+    return hip_check(c, e, "ffa_sort_particles");
+}
+
 int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *z, int64_t n, double *out,
                     int32_t mode, double zt) {
     ffa_ctx_s *c = as_ctx(ctx);

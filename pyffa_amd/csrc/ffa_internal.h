@@ -85,6 +85,11 @@ hipError_t launch_mlp_adam(ffa_ctx_s *c, float *params, const float *grad, float
                            const ffa_adam *hp);
 hipError_t launch_mlp_eval(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *packed, const float *pts, int64_t n,
                            float *out, int mode);
+size_t sort_workspace_bytes(int64_t count);
+hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
+                             const uint32_t **order_out);
+hipError_t launch_gather_particles(ffa_ctx_s *c, const ffa_particles *src, const ffa_particles *dst,
+                                   const uint32_t *order, int64_t count, const int64_t *pid_src, int64_t *pid_dst);
 hipError_t mlp_upload_desc(ffa_ctx_s *c, const ffa_mlp_desc *d);   // -> FFA_CTX_MLP_OFF (synchronous)
 namespace mlp {
 size_t workspace_layout(int64_t n, ffa_mlp_layout &L);

@@ -1,0 +1,208 @@
+// Periodic particle ordering for the FP32 tier (gfx950): a stable sort of the
+// injected particles by azimuth bin, then one gather pass over the state.
+//
+// Particles that sit next to each other in memory then sit next to each
+// other in the machine: the waves of the push read neighbouring Taylor-map
+// cells (the map is periodic over a sector, so an azimuth order over 2 pi is
+// also an order over the sector) and the space-charge gather reads
+// neighbouring slices.  The reference keeps particles in their input order;
+// the order is internal here (the host mirror is rebuilt in the input order
+// from the carried particle ids).
+//
+// Sort: key = floor((fi mod 2 pi) * nbins / 2 pi) for live particles, nbins
+// for the others (they go last), nbins <= 65535.  Two LSD passes over 8-bit
+// digits; each pass is a per-tile digit histogram, one exclusive scan over
+// (digit, tile), and a scatter whose in-tile ranks come from wave ballots in
+// element order, so the result is stable and does not depend on timing.
+#include "ffa_internal.h"
+
+namespace {
+
+constexpr int SBLOCK = 256;
+constexpr int PER_THREAD = 16;
+constexpr int TILE = SBLOCK * PER_THREAD;     // elements per tile (4096)
+constexpr int WAVES = SBLOCK / 64;
+constexpr int PER_WAVE = TILE / WAVES;        // 1024 = 16 chunks of 64
+
+__global__ __launch_bounds__(SBLOCK) void k_sort_keys(const double *__restrict__ fi, const uint8_t *__restrict__ flags,
+                                                     int64_t count, int nbins, uint16_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ idx) {
+    const int64_t i = blockIdx.x * (int64_t)SBLOCK + threadIdx.x;
+    if (i >= count) return;
+    const uint8_t f = flags[i];
+    int key = nbins;
+    if ((f & FFA_F_INJ) && (f & FFA_F_ALIVE)) {
+        const double v = fi[i] * (1.0 / FFA_TWO_PI);
+        const double u = v - floor(v);                   // [0, 1); NaN fails both tests below
+        if (u >= 0.0 && u < 1.0) {
+            key = (int)(u * (double)nbins);
+            if (key >= nbins) key = nbins - 1;
+        }
+    }
+    keys[i] = (uint16_t)key;
+    idx[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_sort_hist(const uint16_t *__restrict__ keys, int64_t count, int shift,
+                                                     int ntiles, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    for (int k = 0; k < PER_THREAD; ++k) {
+        const int64_t e = base + k * SBLOCK + threadIdx.x;
+        if (e < count) atomicAdd(&h[(keys[e] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];      // digit-major
+}
+
+// exclusive scan of 256 * ntiles counters (digit-major), one block
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *__restrict__ hist, int total) {
+    __shared__ uint32_t part[1024];
+    const int per = (total + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(b + per, total);
+    uint32_t s = 0;
+    for (int k = b; k < e; ++k) s += hist[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 1; w < 1024; w <<= 1) {
+        const uint32_t add = threadIdx.x >= w ? part[threadIdx.x - w] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t acc = part[threadIdx.x] - s;
+    for (int k = b; k < e; ++k) {
+        const uint32_t v = hist[k];
+        hist[k] = acc;
+        acc += v;
+    }
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_sort_scatter(const uint16_t *__restrict__ kin,
+                                                        const uint32_t *__restrict__ iin, int64_t count, int shift,
+                                                        int ntiles, const uint32_t *__restrict__ offs,
+                                                        uint16_t *__restrict__ kout, uint32_t *__restrict__ iout) {
+    __shared__ uint32_t cnt[WAVES][256];
+    for (int k = threadIdx.x; k < WAVES * 256; k += SBLOCK) (&cnt[0][0])[k] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)w * PER_WAVE;
+    uint16_t key[PER_THREAD];
+    uint32_t id[PER_THREAD], rank[PER_THREAD];
+    // chunks in element order; ranks count the earlier equal digits of this wave
+    for (int ch = 0; ch < PER_THREAD; ++ch) {
+        const int64_t e = base + ch * 64 + lane;
+        const bool valid = e < count;
+        key[ch] = valid ? kin[e] : (uint16_t)0;
+        id[ch] = valid ? iin[e] : 0u;
+        const int d = (key[ch] >> shift) & 255;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & below);
+        const uint32_t base_cnt = cnt[w][d];
+        rank[ch] = base_cnt + before;
+        // the last lane of each group of equal digits advances the counter
+        if (valid && (peers >> lane) == 1ull) cnt[w][d] = base_cnt + before + 1;
+    }
+    __syncthreads();
+    // per-digit offsets of the waves inside the tile
+    if (threadIdx.x < 256) {
+        uint32_t acc = 0;
+        for (int q = 0; q < WAVES; ++q) {
+            const uint32_t v = cnt[q][threadIdx.x];
+            cnt[q][threadIdx.x] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    for (int ch = 0; ch < PER_THREAD; ++ch) {
+        const int64_t e = base + ch * 64 + lane;
+        if (e >= count) continue;
+        const int d = (key[ch] >> shift) & 255;
+        const uint32_t pos = offs[d * ntiles + blockIdx.x] + cnt[w][d] + rank[ch];
+        kout[pos] = key[ch];
+        iout[pos] = id[ch];
+    }
+}
+
+template <typename ET>
+__global__ __launch_bounds__(SBLOCK) void k_gather_particles(ffa_particles s, ffa_particles d,
+                                                            const uint32_t *__restrict__ order, int64_t count,
+                                                            const int64_t *__restrict__ pid_s,
+                                                            int64_t *__restrict__ pid_d) {
+    const int64_t k = blockIdx.x * (int64_t)SBLOCK + threadIdx.x;
+    if (k >= s.n) return;
+    const int64_t j = k < count ? (int64_t)order[k] : k;
+    const int64_t n = s.n;
+    d.r[k] = s.r[j];
+    d.vr[k] = s.vr[j];
+    d.z[k] = s.z[j];
+    d.vz[k] = s.vz[j];
+    d.fi[k] = s.fi[j];
+    d.fidot[k] = s.fidot[j];
+    d.rf_phase[k] = s.rf_phase[j];
+    d.flags[k] = s.flags[j];
+    const ET *ep = reinterpret_cast<const ET *>(s.esc_pre), *epp = reinterpret_cast<const ET *>(s.esc_prepre);
+    ET *dp = reinterpret_cast<ET *>(d.esc_pre), *dpp = reinterpret_cast<ET *>(d.esc_prepre);
+    for (int c = 0; c < 3; ++c) {
+        dp[c * n + k] = ep[c * n + j];
+        dpp[c * n + k] = epp[c * n + j];
+    }
+    pid_d[k] = pid_s[j];
+}
+
+}  // namespace
+
+int64_t sort_tiles(int64_t count) { return (count + TILE - 1) / TILE; }
+
+size_t sort_workspace_bytes(int64_t count) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int64_t nt = sort_tiles(count);
+    return 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4) + al((size_t)nt * 256 * 4);
+}
+
+hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
+                             const uint32_t **order_out) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char *w = static_cast<char *>(ws);
+    const int64_t nt = sort_tiles(count);
+    uint16_t *k0 = reinterpret_cast<uint16_t *>(w);
+    uint16_t *k1 = reinterpret_cast<uint16_t *>(w + al((size_t)count * 2));
+    uint32_t *i0 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2));
+    uint32_t *i1 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + al((size_t)count * 4));
+    uint32_t *hist = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4));
+    *order_out = i0;
+    if (count == 0) return hipSuccess;
+    const unsigned g = (unsigned)((count + SBLOCK - 1) / SBLOCK);
+    hipLaunchKernelGGL(k_sort_keys, dim3(g), dim3(SBLOCK), 0, c->stream, p->fi, p->flags, count, nbins, k0, i0);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int shift = 8 * pass;
+        uint16_t *ki = pass == 0 ? k0 : k1, *ko = pass == 0 ? k1 : k0;
+        uint32_t *ii = pass == 0 ? i0 : i1, *io = pass == 0 ? i1 : i0;
+        hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, count, shift, (int)nt,
+                           hist);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, c->stream, hist, (int)(256 * nt));
+        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, ii, count, shift,
+                           (int)nt, hist, ko, io);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_particles(ffa_ctx_s *c, const ffa_particles *src, const ffa_particles *dst,
+                                   const uint32_t *order, int64_t count, const int64_t *pid_src, int64_t *pid_dst) {
+    if (src->n == 0) return hipSuccess;
+    const unsigned g = (unsigned)((src->n + SBLOCK - 1) / SBLOCK);
+    if (src->esc_f32)
+        hipLaunchKernelGGL(k_gather_particles<float>, dim3(g), dim3(SBLOCK), 0, c->stream, *src, *dst, order, count,
+                           pid_src, pid_dst);
+    else
+        hipLaunchKernelGGL(k_gather_particles<double>, dim3(g), dim3(SBLOCK), 0, c->stream, *src, *dst, order, count,
+                           pid_src, pid_dst);
+    return hipGetLastError();
+}

@@ -112,9 +112,9 @@ def sync_tracker_from_bunch(tr, B):
     """Make an oracle Tracker continue exactly from a GPU bunch's state:
     state columns, flags, SC endpoint history and masks, injection head."""
     tr.M[:] = B.LocalBunch
-    fl = B.flags.cpu().numpy()
-    tr.E_pre[:] = B.esc_pre.cpu().numpy().astype(np.float64).T
-    tr.E_prepre[:] = B.esc_prepre.cpu().numpy().astype(np.float64).T
+    fl = B.ordered("flags").cpu().numpy()
+    tr.E_pre[:] = B.ordered("esc_pre").cpu().numpy().astype(np.float64).T
+    tr.E_prepre[:] = B.ordered("esc_prepre").cpu().numpy().astype(np.float64).T
     tr.m_pre[:] = (fl & 4) != 0
     tr.m_prepre[:] = (fl & 8) != 0
     tr.head = B._uninject_head
