@@ -149,6 +149,8 @@ typedef struct {
     size_t records;     /* uint64 n: slice-sorted packed CIC records */
     size_t slice_start; /* int32 ns + 1: record offset of each slice */
     size_t cursor;      /* int32 ns: fill cursors */
+    size_t records_by_row;  /* uint64 n: records of each slice ordered by CIC row */
+    size_t row_start;   /* int32 ns * (nz + 1): record offset of each (slice, row) */
 } ffa_sc_layout;
 
 #define FFA_SC_BLOCKS 1024

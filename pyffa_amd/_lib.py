@@ -80,7 +80,7 @@ class FfaScResult(ctypes.Structure):
 class FfaScLayout(ctypes.Structure):
     _fields_ = [(name, ctypes.c_size_t) for name in (
         "total", "grid", "cell_count", "cbuf", "phi", "ex", "ez", "px", "ppr0", "pslice",
-        "slice_count", "partials", "result", "records", "slice_start", "cursor")]
+        "slice_count", "partials", "result", "records", "slice_start", "cursor", "records_by_row", "row_start")]
 
 
 MLP_NFREQ = 32

@@ -105,6 +105,7 @@ int ffa_set_machine(void *ctx, const ffa_machine *m) {
     if (m->n_harm < 0 || m->n_harm > FFA_MAX_HARM) return fail(c, FFA_ERR_INVALID, "n_harm out of range");
     if (m->n_gap < 0 || m->n_gap > FFA_MAX_GAP) return fail(c, FFA_ERR_INVALID, "n_gap out of range");
     if (m->max_order < 0) return fail(c, FFA_ERR_INVALID, "max_order negative");
+    if (m->taylor_f32 < 0 || m->taylor_f32 > 1) return fail(c, FFA_ERR_INVALID, "taylor_f32 must be 0 or 1");
     if (m->max_order > 0) {
         if (!m->taylor) return fail(c, FFA_ERR_INVALID, "max_order > 0 but no Taylor map");
         if (m->r_size < 2 || m->fi_size < 2) return fail(c, FFA_ERR_INVALID, "Taylor map needs >= 2 nodes per axis");

@@ -25,7 +25,7 @@ constexpr int BLOCK = 256;
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
 __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float z,
-                                           float xf, float yf, float v0, float rf_red, float &Bz, float &Br,
+                                           float xf, float yf, float v0, double rf, float &Bz, float &Br,
                                            float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
                                            bool taylor = true) {
     // fast_mod on the branch-cut azimuth (equal to the unwrapped one mod the sector)
@@ -67,10 +67,10 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
         const float tx = fx - (float)i, ty = fy - (float)j;
         const float w00 = (1.0f - tx) * (1.0f - ty), w01 = (1.0f - tx) * ty, w10 = tx * (1.0f - ty), w11 = tx * ty;
         const int sj = 3 * k->no;
+        const float z2 = z * z;
         // (i, j), (i, j+1) and (i+1, j), (i+1, j+1): two contiguous runs of 2 sj floats
         const float2 *a0 = reinterpret_cast<const float2 *>(k->T + ((int64_t)i * k->NY + j) * sj);
         const float2 *a1 = reinterpret_cast<const float2 *>(k->T + ((int64_t)(i + 1) * k->NY + j) * sj);
-        const float z2 = z * z;
         float ze = z2, zo = z2 * z;
         if (k->no == 2) {
             // 12 floats per run: order1 (Bz,Bfi,Br), order2 (...), for j and j+1
@@ -139,7 +139,11 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
                 gap = 0;
             }
         }
-        sin_rf = __sinf(rf_red + k->rf_shift[gap]);
+        // the RF factor only matters inside a gap (E = 0 elsewhere)
+        if (Er != 0.0f || Efi != 0.0f) {
+            const double rfr = rf - FFA_TWO_PI * rint(rf * (1.0 / FFA_TWO_PI));
+            sin_rf = __sinf((float)rfr + k->rf_shift[gap]);
+        }
     }
 }
 
@@ -222,9 +226,8 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
         const float xf = (float)x, yf = (float)y, zf = (float)z;
         const float rr = sqrtf(xf * xf + yf * yf);
         const float th = atan2f(yf, xf);
-        const double rfr = rf - FFA_TWO_PI * rint(rf * (1.0 / FFA_TWO_PI));
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
-        fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, (float)rfr, Bz, Br, Bfi, Er, Efi, Ez, srf,
+        fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
                    SUR ? (!use_sur && md->fallback != 0) : true);
         if constexpr (SUR) {
             Bz += sBz;
@@ -247,7 +250,11 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
         const double ig = rsqrt(1.0 + (mx * mx + my * my + mz * mz) * inv_c2);
         const double tx = eps * ig * Bx, ty = eps * ig * By, tz = eps * ig * Bzc;
         const double pxp = mx + (my * tz - mz * ty), pyp = my + (mz * tx - mx * tz), pzp = mz + (mx * ty - my * tx);
-        const double sc2 = 2.0 / (1.0 + tx * tx + ty * ty + tz * tz);
+        // 2 / (1 + t^2): |t| ~ omega dt / 2 is small, so the series to t^6
+        // (relative error <= t^8 <= 1e-16) replaces the FP64 division
+        const double t2 = tx * tx + ty * ty + tz * tz;
+        double sc2 = 2.0 * (1.0 - t2 * (1.0 - t2 * (1.0 - t2)));
+        if (t2 >= 1e-4) sc2 = 2.0 / (1.0 + t2);
         ux = mx + sc2 * (pyp * tz - pzp * ty) + eps * Ex;
         uy = my + sc2 * (pzp * tx - pxp * tz) + eps * Ey;
         uz = mz + sc2 * (pxp * ty - pyp * tx) + eps * Ezc;

@@ -12,6 +12,7 @@
 //   k_sc_bin           per survivor: CIC cell + fixed-point weights packed in
 //                      one 64-bit record, written slice-sorted (two-level
 //                      reservation: one global atomic per block and slice)
+//   k_sc_rowsort       per slice: records ordered by CIC row, row offsets
 //   k_sc_deposit_tiles one workgroup per (slice, z band): LDS int64 / int32
 //                      accumulation of the slice's records, plain stores of
 //                      the band (integer sums: bit-identical for any order,
@@ -305,11 +306,57 @@ __global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params
     }
 }
 
+// One workgroup per slice: order the slice's records by CIC row iz (LDS
+// counting sort; the order inside a row is immaterial, the deposit sums
+// integers) and record where each row starts, so a band of rows is one
+// contiguous range of records.
+constexpr int ROW_BLOCK = 1024;
+
+__global__ __launch_bounds__(ROW_BLOCK) void k_sc_rowsort(ffa_sc_params sp, const ffa_sc_result *res,
+                                                         const int32_t *slice_start, const uint64_t *rec,
+                                                         uint64_t *rec2, int32_t *row_start) {
+    __shared__ int32_t cnt[1024];
+    __shared__ int32_t tot[ROW_BLOCK];
+    const int nz = sp.nz, s = blockIdx.x;
+    int32_t *rs = row_start + (size_t)s * (nz + 1);
+    const int r0 = slice_start[s], r1 = slice_start[s + 1];
+    const bool ok = res->bad_grid == 0.0;
+    for (int k = threadIdx.x; k < nz; k += ROW_BLOCK) cnt[k] = 0;
+    __syncthreads();
+    if (ok)
+        for (int t = r0 + threadIdx.x; t < r1; t += ROW_BLOCK) atomicAdd(&cnt[(int)((rec[t] >> 10) & 1023u)], 1);
+    __syncthreads();
+    // exclusive scan of the nz <= 1024 row counts
+    const int own = threadIdx.x < nz ? cnt[threadIdx.x] : 0;
+    tot[threadIdx.x] = own;
+    __syncthreads();
+    for (int w = 1; w < ROW_BLOCK; w <<= 1) {
+        const int add = threadIdx.x >= w ? tot[threadIdx.x - w] : 0;
+        __syncthreads();
+        tot[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const int excl = tot[threadIdx.x] - own;
+    __syncthreads();
+    if (threadIdx.x < nz) {
+        cnt[threadIdx.x] = r0 + excl;
+        rs[threadIdx.x] = r0 + excl;
+    }
+    if (threadIdx.x == 0) rs[nz] = ok ? r1 : r0;
+    __syncthreads();
+    if (ok)
+        for (int t = r0 + threadIdx.x; t < r1; t += ROW_BLOCK) {
+            const uint64_t w = rec[t];
+            rec2[atomicAdd(&cnt[(int)((w >> 10) & 1023u)], 1)] = w;
+        }
+}
+
 // One workgroup per (slice, band of bz rows): zero an LDS tile, add the
-// corners of every record of the slice that fall in the band, store the
-// band.  Dynamic LDS: bz * nx int64 sums, then bz * nx int32 base-cell counts.
+// corners of the records of rows z0 - 1 .. z1 - 1 that fall in the band (a
+// contiguous range after k_sc_rowsort), store the band.  Dynamic LDS:
+// bz * nx int64 sums, then bz * nx int32 base-cell counts.
 __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, const ffa_sc_result *res,
-                                                           const int32_t *slice_start, const uint64_t *rec,
+                                                           const int32_t *row_start, const uint64_t *rec,
                                                            unsigned long long *grid, int32_t *cell, int bz) {
     extern __shared__ unsigned long long lds[];
     const int nx = sp.nx, nz = sp.nz;
@@ -323,7 +370,8 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, co
     __syncthreads();
     if (res->bad_grid == 0.0) {
         const uint64_t one = (uint64_t)1 << sp.bits;
-        const int r0 = slice_start[s], r1 = slice_start[s + 1];
+        const int32_t *rs = row_start + (size_t)s * (nz + 1);
+        const int r0 = rs[z0 > 0 ? z0 - 1 : 0], r1 = rs[z1];
         for (int t = r0 + threadIdx.x; t < r1; t += BLOCK) {
             const uint64_t w = rec[t];
             const int ix = (int)(w & 1023u), iz = (int)((w >> 10) & 1023u);
@@ -367,16 +415,19 @@ __device__ __forceinline__ void make_twiddles(float2 *tw, int N, bool inv) {
     }
 }
 
-// returns the buffer holding the result
-__device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, int N, int logN) {
+// In-LDS radix-2 Stockham FFT of pts / N lines of N points; returns the
+// buffer holding the result.
+// Lines are ls >= N elements apart (padding against LDS bank conflicts).
+__device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, int N, int logN, int pts = FFT_PTS, int ls = 0) {
     const int half = N >> 1;
-    const int nbf = (FFT_PTS >> 1);   // butterflies per pass over all lines
+    const int nbf = (pts >> 1);       // butterflies per pass over all lines
+    if (ls == 0) ls = N;
     for (int sl = 0; sl < logN; ++sl) {
         const int s = 1 << sl, m = N >> (sl + 1);
         for (int t = threadIdx.x; t < nbf; t += BLOCK) {
             const int line = t / half, b = t - line * half;
             const int pp = b >> sl, q = b & (s - 1);
-            float2 *xl = x + line * N, *yl = y + line * N;
+            float2 *xl = x + line * ls, *yl = y + line * ls;
             const float2 a = xl[q + s * pp], bb = xl[q + s * (pp + m)];
             const float2 w = tw[pp * s];
             yl[q + s * (2 * pp)] = make_float2(a.x + bb.x, a.y + bb.y);
@@ -413,28 +464,36 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(const int64_t *grid, flo
 
 // columns: forward FFT along z, multiply by the periodic Green's function
 // (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760), inverse FFT.
+// Columns per workgroup: COL_PTS / nz (at most nx).  Wider tiles (4096
+// points) measured slower at nz = 256: fewer workgroups, lower occupancy.
+constexpr int COL_PTS = 1024;
+
+__host__ __device__ inline int col_width(int nx, int nz) { return COL_PTS / nz < nx ? COL_PTS / nz : nx; }
+
 __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const ffa_sc_result *res,
                                                          int lognz) {
-    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2], twi[FFT_PTS / 2];
+    __shared__ float2 b0[COL_PTS + 32], b1[COL_PTS + 32], tw[512], twi[512];
     const int nx = sp.nx, nz = sp.nz;
-    const int W = FFT_PTS / nz;                       // columns per block
+    const int W = col_width(nx, nz);                  // columns per block
+    const int PTS = W * nz;
+    const int LS = nz + 1;                            // padded column stride in LDS
     const int cpp = nx / W;                           // column groups per plane
     const int64_t plane = blockIdx.x / cpp;
     const int c0 = (blockIdx.x - plane * cpp) * W;
     float2 *base = cbuf + plane * (int64_t)nz * nx;
     make_twiddles(tw, nz, false);
     make_twiddles(twi, nz, true);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+    for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
-        b0[c * nz + j] = base[(int64_t)j * nx + c0 + c];
+        b0[c * LS + j] = base[(int64_t)j * nx + c0 + c];
     }
     __syncthreads();
-    float2 *o = lds_fft(b0, b1, tw, nz, lognz);
+    float2 *o = lds_fft(b0, b1, tw, nz, lognz, PTS, LS);
     float2 *other = (o == b0) ? b1 : b0;
     // Green's function in the reference's float32 arithmetic
     const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
     const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+    for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int c = e / nz, j = e - c * nz;
         const int ci = c0 + c;
         const int idx = ci <= nx / 2 ? ci : ci - nx;
@@ -443,13 +502,14 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
         const float kz2 = kz * kz;
         const float k2 = kx * kx + kz2;
         const float g = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
-        o[e] = make_float2(o[e].x * g, o[e].y * g);
+        float2 &v = o[c * LS + j];
+        v = make_float2(v.x * g, v.y * g);
     }
     __syncthreads();
-    float2 *r = lds_fft(o, other, twi, nz, lognz);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
+    float2 *r = lds_fft(o, other, twi, nz, lognz, PTS, LS);
+    for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
-        base[(int64_t)j * nx + c0 + c] = r[c * nz + j];
+        base[(int64_t)j * nx + c0 + c] = r[c * LS + j];
     }
 }
 
@@ -600,6 +660,8 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     L->records = o; o = al(o + 8 * (size_t)n);
     L->slice_start = o; o = al(o + 4 * ((size_t)ns + 1));
     L->cursor = o; o = al(o + 4 * (size_t)ns);
+    L->records_by_row = o; o = al(o + 8 * (size_t)n);
+    L->row_start = o; o = al(o + 4 * (size_t)ns * ((size_t)nz + 1));
     L->total = o;
     return 0;
 }
@@ -634,14 +696,18 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, cursor, rec);
     const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
     const size_t tile_lds = (size_t)bz * sp->nx * 12;
+    uint64_t *rec2 = reinterpret_cast<uint64_t *>(w + L->records_by_row);
+    int32_t *rstart = reinterpret_cast<int32_t *>(w + L->row_start);
+    hipLaunchKernelGGL(k_sc_rowsort, dim3((unsigned)sp->ns), dim3(ROW_BLOCK), 0, st, *sp, res, sstart, rec, rec2,
+                       rstart);
     hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st, *sp,
-                       res, sstart, rec, reinterpret_cast<unsigned long long *>(grid), cell, bz);
+                       res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
     const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
     hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, grid, cbuf, *sp,
                        res, lognx);
-    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / (FFT_PTS / sp->nz))), dim3(BLOCK), 0, st,
+    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK), 0, st,
                        cbuf, *sp, res, lognz);
     hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, cbuf, phi, *sp,
                        lognx);

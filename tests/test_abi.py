@@ -28,7 +28,7 @@ def test_struct_sizes_match_header_layout():
     # ffa_step: 5 doubles + 3 + 3
     assert ctypes.sizeof(_lib.FfaStep) == 11 * 8
     assert ctypes.sizeof(_lib.FfaParticles) == 10 * 8 + 8 + 8
-    assert ctypes.sizeof(_lib.FfaScLayout) == 16 * 8
+    assert ctypes.sizeof(_lib.FfaScLayout) == 18 * 8
     assert ctypes.sizeof(_lib.FfaScResult) == 32 * 8
     assert ctypes.sizeof(_lib.FfaMlpDesc) == 8 * 8 + 3 * 32 * 4 + 8
     assert ctypes.sizeof(_lib.FfaAdam) == 6 * 4
@@ -57,7 +57,7 @@ def test_mlp_workspace_layout():
 def test_workspace_layout_is_aligned_and_ordered():
     L = _lib.sc_layout(1000, 33, 64, 32)
     offs = [L.grid, L.cell_count, L.cbuf, L.phi, L.ex, L.ez, L.px, L.ppr0, L.pslice, L.slice_count, L.partials,
-            L.result, L.records, L.slice_start, L.cursor]
+            L.result, L.records, L.slice_start, L.cursor, L.records_by_row, L.row_start]
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.total > offs[-1]
     assert L.cell_count - L.grid >= 8 * 33 * 64 * 32
