@@ -407,58 +407,87 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// w^k = exp(-+ 2 pi i k / N) for k < N (the radix-4 stages use exponents up
+// to 3N/4)
 __device__ __forceinline__ void make_twiddles(float2 *tw, int N, bool inv) {
-    for (int k = threadIdx.x; k < N / 2; k += BLOCK) {
+    for (int k = threadIdx.x; k < N; k += BLOCK) {
         double s, c;
         sincospi(-2.0 * (double)k / (double)N, &s, &c);
         tw[k] = make_float2((float)c, (float)(inv ? -s : s));
     }
 }
 
-// In-LDS radix-2 Stockham FFT of pts / N lines of N points; returns the
-// buffer holding the result.
-// Lines are ls >= N elements apart (padding against LDS bank conflicts).
-__device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, int N, int logN, int pts = FFT_PTS, int ls = 0) {
-    const int half = N >> 1;
-    const int nbf = (pts >> 1);       // butterflies per pass over all lines
+// In-LDS Stockham FFT of pts / N lines of N points: radix-4 stages
+//   a_j = x[q + s (p + j m)], j < 4, m = N / (4 s)
+//   y[q + s (4 p + k)] = w^(k p s) sum_j a_j (-+i)^(j k)
+// then one radix-2 stage when log2 N is odd; returns the buffer holding the
+// result.  Lines are ls >= N elements apart (padding against bank conflicts).
+__device__ float2 *lds_fft4(float2 *x, float2 *y, const float2 *tw, int N, int logN, bool inv, int pts = FFT_PTS,
+                            int ls = 0) {
     if (ls == 0) ls = N;
-    for (int sl = 0; sl < logN; ++sl) {
-        const int s = 1 << sl, m = N >> (sl + 1);
-        for (int t = threadIdx.x; t < nbf; t += BLOCK) {
-            const int line = t / half, b = t - line * half;
-            const int pp = b >> sl, q = b & (s - 1);
-            float2 *xl = x + line * ls, *yl = y + line * ls;
-            const float2 a = xl[q + s * pp], bb = xl[q + s * (pp + m)];
-            const float2 w = tw[pp * s];
-            yl[q + s * (2 * pp)] = make_float2(a.x + bb.x, a.y + bb.y);
-            yl[q + s * (2 * pp + 1)] = cmul(make_float2(a.x - bb.x, a.y - bb.y), w);
+    const int qn = N >> 2;
+    int s = 1, logs = 0;
+    for (; 4 * s <= N; s <<= 2, logs += 2) {
+        const int m = N / (4 * s);
+        for (int t = threadIdx.x; t < (pts >> 2); t += BLOCK) {
+            const int line = t / qn, b = t - line * qn;
+            const int pp = b >> logs, q = b & (s - 1);
+            const float2 *xl = x + line * ls;
+            float2 *yl = y + line * ls;
+            const float2 a0 = xl[q + s * pp], a1 = xl[q + s * (pp + m)];
+            const float2 a2 = xl[q + s * (pp + 2 * m)], a3 = xl[q + s * (pp + 3 * m)];
+            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y), t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
+            const float2 d = make_float2(a1.x - a3.x, a1.y - a3.y);
+            const float2 t3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);     // (+-i) d
+            const int e = pp * s;
+            float2 *o = yl + q + s * (4 * pp);
+            o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+            o[s] = cmul(make_float2(t1.x + t3.x, t1.y + t3.y), tw[e]);
+            o[2 * s] = cmul(make_float2(t0.x - t2.x, t0.y - t2.y), tw[2 * e]);
+            o[3 * s] = cmul(make_float2(t1.x - t3.x, t1.y - t3.y), tw[3 * e]);
         }
         __syncthreads();
         float2 *tmp = x; x = y; y = tmp;
     }
+    if (s < N) {                        // radix 2, m = N / (2 s)
+        const int m = N / (2 * s), half = N >> 1;
+        for (int t = threadIdx.x; t < (pts >> 1); t += BLOCK) {
+            const int line = t / half, b = t - line * half;
+            const int pp = b >> logs, q = b & (s - 1);
+            const float2 *xl = x + line * ls;
+            float2 *yl = y + line * ls;
+            const float2 a = xl[q + s * pp], bb = xl[q + s * (pp + m)];
+            yl[q + s * (2 * pp)] = make_float2(a.x + bb.x, a.y + bb.y);
+            yl[q + s * (2 * pp + 1)] = cmul(make_float2(a.x - bb.x, a.y - bb.y), tw[pp * s]);
+        }
+        __syncthreads();
+        float2 *tmp = x; x = y; y = tmp;
+    }
+    (void)logN;
     return x;
 }
 
 // rows: forward FFT along x of the packed slice pair; grid -> cbuf.
 __global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(const int64_t *grid, float2 *cbuf, ffa_sc_params sp,
-                                                       const ffa_sc_result *res, int lognx) {
-    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2];
+                                                       const ffa_sc_result *res, int lognx, int lognz) {
+    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS];
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
     const int lines = FFT_PTS / nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
     const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
     make_twiddles(tw, nx, false);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
-        const int l = e / nx, i = e - l * nx;
+        const int l = e >> lognx, i = e & (nx - 1);
         const int64_t row = row0 + l;
-        const int64_t plane = row / nz, j = row - plane * nz;
+        const int64_t plane = row >> lognz, j = row & (nz - 1);
         const int64_t sa = 2 * plane, sb = sa + 1;
         const float re = (float)((double)grid[(sa * nz + j) * nx + i] * us);
         const float im = sb < ns ? (float)((double)grid[(sb * nz + j) * nx + i] * us) : 0.0f;
         b0[e] = make_float2(re, im);
     }
     __syncthreads();
-    const float2 *o = lds_fft(b0, b1, tw, nx, lognx);
+    const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, false);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) cbuf[row0 * nx + e] = o[e];
 }
 
@@ -472,7 +501,7 @@ __host__ __device__ inline int col_width(int nx, int nz) { return COL_PTS / nz <
 
 __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const ffa_sc_result *res,
                                                          int lognz) {
-    __shared__ float2 b0[COL_PTS + 32], b1[COL_PTS + 32], tw[512], twi[512];
+    __shared__ float2 b0[COL_PTS + 32], b1[COL_PTS + 32], tw[1024], twi[1024];
     const int nx = sp.nx, nz = sp.nz;
     const int W = col_width(nx, nz);                  // columns per block
     const int PTS = W * nz;
@@ -488,7 +517,7 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
         b0[c * LS + j] = base[(int64_t)j * nx + c0 + c];
     }
     __syncthreads();
-    float2 *o = lds_fft(b0, b1, tw, nz, lognz, PTS, LS);
+    float2 *o = lds_fft4(b0, b1, tw, nz, lognz, false, PTS, LS);
     float2 *other = (o == b0) ? b1 : b0;
     // Green's function in the reference's float32 arithmetic
     const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
@@ -506,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
         v = make_float2(v.x * g, v.y * g);
     }
     __syncthreads();
-    float2 *r = lds_fft(o, other, twi, nz, lognz, PTS, LS);
+    float2 *r = lds_fft4(o, other, twi, nz, lognz, true, PTS, LS);
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
         base[(int64_t)j * nx + c0 + c] = r[c * LS + j];
@@ -514,20 +543,21 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
 }
 
 // rows: inverse FFT along x, normalise by 1/(nx nz), unpack the slice pair.
-__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp, int lognx) {
-    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS / 2];
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp, int lognx,
+                                                       int lognz) {
+    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS];
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
     const int lines = FFT_PTS / nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;
     make_twiddles(tw, nx, true);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
     __syncthreads();
-    const float2 *o = lds_fft(b0, b1, tw, nx, lognx);
+    const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, true);
     const float norm = 1.0f / (float)(nx * nz);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
-        const int l = e / nx, i = e - l * nx;
+        const int l = e >> lognx, i = e & (nx - 1);
         const int64_t row = row0 + l;
-        const int64_t plane = row / nz, j = row - plane * nz;
+        const int64_t plane = row >> lognz, j = row & (nz - 1);
         const int64_t sa = 2 * plane, sb = sa + 1;
         phi[(sa * nz + j) * nx + i] = o[e].x * norm;
         if (sb < ns) phi[(sb * nz + j) * nx + i] = o[e].y * norm;
@@ -537,14 +567,14 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, floa
 // compute_field_from_potential_numba_SBS (src/FFAG_SC_new.py:672-728):
 // float32 differences, float64 scaling, float32 store.
 __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float *Ex, float *Ez, ffa_sc_params sp,
-                                                   const ffa_sc_result *res) {
+                                                   const ffa_sc_result *res, int lognx, int lognz) {
     const int nx = sp.nx, nz = sp.nz;
     const int64_t total = (int64_t)sp.ns * nz * nx;
     const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
     if (t >= total) return;
-    const int64_t k = t / ((int64_t)nz * nx);
-    const int rem = (int)(t - k * (int64_t)nz * nx);
-    const int j = rem / nx, i = rem - j * nx;
+    const int64_t k = t >> (lognx + lognz);                 // nx, nz are powers of two
+    const int rem = (int)(t & (((int64_t)nz << lognx) - 1));
+    const int j = rem >> lognx, i = rem & (nx - 1);
     const double dr = res->dx, dz = res->dz, bl = res->bunch_length;
     if (res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0) { Ex[t] = 0.0f; Ez[t] = 0.0f; return; }
     const double gamma = res->mean_gamma_m1 + 1.0;
@@ -593,7 +623,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             int s = pslice[i];
             if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
             const float *Ex = Exc + (size_t)s * nz * nx, *Ez = Ezc + (size_t)s * nz * nx;
-            const double t = (px[i] - res->xmin) / res->dx, u = (p.z[i] - res->zmin) / res->dz;
+            const double t = (px[i] - res->xmin) * (1.0 / res->dx), u = (p.z[i] - res->zmin) * (1.0 / res->dz);
             const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
             int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cu);
             if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
@@ -604,9 +634,10 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             const double er = w00 * Ex[b] + w10 * Ex[b + 1] + w01 * Ex[b + nx] + w11 * Ex[b + nx + 1];
             const double ez = w00 * Ez[b] + w10 * Ez[b + 1] + w01 * Ez[b + nx] + w11 * Ez[b + nx + 1];
             const double pr = ppr0[i];
-            nr = er * (1.0 / sqrt(1.0 + pr * pr));
+            const double ip = 1.0 / sqrt(1.0 + pr * pr);
+            nr = er * ip;
             nz_ = ez;
-            nf = er * (-pr / sqrt(1.0 + pr * pr));
+            nf = er * (-pr * ip);
         } else if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
             // degenerate grid: the reference leaves LocalBunch columns 10-12 as
             // the previous step wrote them (the extrapolated field)
@@ -706,12 +737,12 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
     hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, grid, cbuf, *sp,
-                       res, lognx);
+                       res, lognx, lognz);
     hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK), 0, st,
                        cbuf, *sp, res, lognz);
     hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, cbuf, phi, *sp,
-                       lognx);
-    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)C)), dim3(BLOCK), 0, st, phi, ex, ez, *sp, res);
+                       lognx, lognz);
+    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)C)), dim3(BLOCK), 0, st, phi, ex, ez, *sp, res, lognx, lognz);
     if (p->n > 0) {
         if (p->esc_f32)
             hipLaunchKernelGGL(k_sc_gather_roll<float>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, ex, ez,
