@@ -160,7 +160,7 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
                                                       const int64_t *__restrict__ inj, int nsteps,
                                                       const char *__restrict__ sur_w,
                                                       const mlp::fdesc *__restrict__ md) {
-    const int64_t i = blockIdx.x * (int64_t)(SUR ? SUR_BLOCK : BLOCK) + threadIdx.x;
+    const int64_t i = xcd_block() * (int64_t)(SUR ? SUR_BLOCK : BLOCK) + threadIdx.x;
     __shared__ __attribute__((aligned(16))) char lds[SUR ? mlp::FWD_BYTES : 16];
     if constexpr (SUR) {
         for (int k = threadIdx.x; k < mlp::FWD_BYTES / 16; k += SUR_BLOCK)

@@ -61,6 +61,18 @@ static constexpr int FFA_MAX_FUSED = (int)(FFA_CTX_STEPS_BYTES / sizeof(ffa_step
 
 static constexpr size_t FFA_CTX_FCONST_OFF = 6144;   // after the machine (< 6 KB)
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs (each with its own
+// L2).  This maps the hardware block index to a logical one so that every XCD
+// works on one contiguous range of logical blocks: neighbouring particles
+// (ordered by azimuth) then share one L2 for their map and field reads.  A
+// bijection on [0, gridDim.x) whatever the dispatch order.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned x = b & 7u, k = b >> 3;
+    const unsigned q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + k;
+}
+
 inline const ffa_machine *ctx_machine_dev(ffa_ctx_s *c) {
     return reinterpret_cast<const ffa_machine *>(c->dev + FFA_CTX_MACHINE_OFF);
 }

@@ -608,7 +608,7 @@ template <typename ET>
 __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
                                                          const float *Exc, const float *Ezc, const double *px,
                                                          const double *ppr0, const int32_t *pslice) {
-    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    const int64_t i = xcd_block() * (int64_t)BLOCK + threadIdx.x;
     if (i >= p.n) return;
     const int64_t n = p.n;
     ET *ep = reinterpret_cast<ET *>(p.esc_pre);
