@@ -216,6 +216,16 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
  * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);
 
+/* Position detectors: detect_azimuth_crossing (src/FFAG_MathTools.py:
+ * 1633-1695) as PositionDump / PositionDumpBunch use it
+ * (src/FFAG_dump.py:610-626, 893-908).  A particle alive before the step
+ * (flags_pre, the reference's Step_SurviveFlag) crosses target in [0, 2 pi)
+ * when its pre/post azimuths, reduced mod 2 pi, bracket target (wrapping
+ * steps included).  Writes the crossing particles' positions to idx_out (n
+ * int64 capacity, arbitrary order) and their number to *count (device int64). */
+int ffa_detect_crossing(void *ctx, const double *fi_pre, const uint8_t *flags_pre, const double *fi_post, int64_t n,
+                        double target, int64_t *idx_out, int64_t *count);
+
 /* Particle order (no counterpart in the reference, which keeps the input
  * order): a stable sort of particles [0, count) by azimuth bin
  * floor((fi mod 2 pi) * nbins / 2 pi) (particles not injected-and-alive

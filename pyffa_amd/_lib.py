@@ -118,7 +118,7 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_push_boris_fused_f32",
             "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
-            "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles")
+            "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing")
 
 
 def load():
@@ -155,6 +155,7 @@ def load():
                                        ctypes.POINTER(FfaAdam), _vp]
     lib.ffa_mlp_eval.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp, _vp, _i64, _vp, _i32]
     lib.ffa_mlp_set.argtypes = [_vp, ctypes.POINTER(FfaMlpDesc), _vp]
+    lib.ffa_detect_crossing.argtypes = [_vp, _vp, _vp, _vp, _i64, _d, _vp, _vp]
     lib.ffa_sort_workspace_size.argtypes = [_i64, ctypes.POINTER(ctypes.c_size_t)]
     lib.ffa_sort_particles.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaParticles), _i64, _i32,
                                        _vp, _vp, _vp]

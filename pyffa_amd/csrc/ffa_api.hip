@@ -239,6 +239,18 @@ int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {
     return hip_check(c, launch_reduce_phi(c, p, out_dev), "ffa_reduce_phi");
 }
 
+int ffa_detect_crossing(void *ctx, const double *fi_pre, const uint8_t *flags_pre, const double *fi_post, int64_t n,
+                        double target, int64_t *idx_out, int64_t *count) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !count) return fail(c, FFA_ERR_INVALID, "ctx or count is NULL");
+    if (n < 0) return fail(c, FFA_ERR_INVALID, "negative particle count");
+    if (n > 0 && (!fi_pre || !flags_pre || !fi_post || !idx_out))
+        return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (!(target >= 0.0 && target < FFA_TWO_PI)) return fail(c, FFA_ERR_INVALID, "target azimuth must be in [0, 2 pi)");
+    return hip_check(c, launch_detect_crossing(c, fi_pre, flags_pre, fi_post, n, target, idx_out, count),
+                     "ffa_detect_crossing");
+}
+
 int ffa_sort_workspace_size(int64_t count, size_t *bytes) {
     if (!bytes) return fail(nullptr, FFA_ERR_INVALID, "bytes is NULL");
     if (count < 0 || count > 0xFFFFFFFFLL) return fail(nullptr, FFA_ERR_INVALID, "count must be in [0, 2^32)");

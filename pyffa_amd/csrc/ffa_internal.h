@@ -97,6 +97,8 @@ hipError_t launch_mlp_adam(ffa_ctx_s *c, float *params, const float *grad, float
                            const ffa_adam *hp);
 hipError_t launch_mlp_eval(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *packed, const float *pts, int64_t n,
                            float *out, int mode);
+hipError_t launch_detect_crossing(ffa_ctx_s *c, const double *fi_pre, const uint8_t *flags_pre, const double *fi_post,
+                                  int64_t n, double target, int64_t *idx_out, int64_t *count);
 size_t sort_workspace_bytes(int64_t count);
 hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
                              const uint32_t **order_out);
