@@ -129,6 +129,9 @@ class FFAG_Bunch:
         self._sort_ws = None
         self._meta = M[:, [6, 13, 14, 15]].copy()
         self._host_cache = None
+        self._pre_fi = self._pre_flags = None
+        self._pre_valid = False
+        self._det_idx = self._det_count = None
 
     _STATE = ("r", "vr", "z", "vz", "fi", "fidot", "rf_phase", "flags", "esc_pre", "esc_prepre", "pid")
 
@@ -170,6 +173,7 @@ class FFAG_Bunch:
             setattr(self, k, alt[k])
         self._alt = cur
         self._host_cache = None
+        self._pre_valid = False         # recorded pre-step arrays follow the old order
 
     def ordered(self, name):
         """Device copy of a state array in the input particle order."""
@@ -184,28 +188,59 @@ class FFAG_Bunch:
     def invalidate_host(self):
         self._host_cache = None
 
+    def _assemble(self, pos=None):
+        """(k, 16) host matrix of the particles held at device positions
+        ``pos`` (an int64 device tensor), or of all particles in the input
+        order when ``pos`` is None."""
+        t = self.torch
+        if pos is None:
+            take = self.ordered
+            meta = self._meta
+        else:
+            def take(name):
+                a = getattr(self, name)
+                return a[:, pos] if a.dim() == 2 else a[pos]
+            meta = self._meta[self.pid[pos].cpu().numpy()]
+        M = np.zeros((meta.shape[0], 16))
+        for j, name in ((0, "r"), (1, "vr"), (2, "z"), (3, "vz"), (4, "fi"), (5, "fidot"), (9, "rf_phase")):
+            M[:, j] = take(name).cpu().numpy()
+        fl = take("flags").cpu().numpy()
+        M[:, 7] = (fl & _lib.F_INJ) != 0
+        M[:, 8] = (fl & _lib.F_ALIVE) != 0
+        M[:, [6, 13, 14, 15]] = meta
+        pre = take("esc_pre").to(t.float64).cpu().numpy()
+        pp = take("esc_prepre").to(t.float64).cpu().numpy()
+        both = ((fl & _lib.F_SCPRE) != 0) & ((fl & _lib.F_SCPREPRE) != 0)
+        esc = pre.copy()
+        esc[:, both] = pre[:, both] + self.last_sc_s * (pre[:, both] - pp[:, both])
+        M[:, 10:13] = esc.T
+        return M
+
     @property
     def LocalBunch(self):
         """(N, 16) host mirror of the device state (read-only view of a copy;
         call ``set_local_bunch`` to write a modified matrix back)."""
         if self._host_cache is None:
-            t = self.torch
-            n = self.TotalParticleNum_Local
-            M = np.zeros((n, 16))
-            for j, name in ((0, "r"), (1, "vr"), (2, "z"), (3, "vz"), (4, "fi"), (5, "fidot"), (9, "rf_phase")):
-                M[:, j] = self.ordered(name).cpu().numpy()
-            fl = self.ordered("flags").cpu().numpy()
-            M[:, 7] = (fl & _lib.F_INJ) != 0
-            M[:, 8] = (fl & _lib.F_ALIVE) != 0
-            M[:, [6, 13, 14, 15]] = self._meta
-            pre = self.ordered("esc_pre").to(t.float64).cpu().numpy()
-            pp = self.ordered("esc_prepre").to(t.float64).cpu().numpy()
-            both = ((fl & _lib.F_SCPRE) != 0) & ((fl & _lib.F_SCPREPRE) != 0)
-            esc = pre.copy()
-            esc[:, both] = pre[:, both] + self.last_sc_s * (pre[:, both] - pp[:, both])
-            M[:, 10:13] = esc.T
-            self._host_cache = M
+            self._host_cache = self._assemble()
         return self._host_cache
+
+    def rows(self, input_rows):
+        """Rows ``input_rows`` (indices in the input order) of the host
+        mirror, gathered on the device so only those rows are copied."""
+        t = self.torch
+        idx = t.as_tensor(np.asarray(input_rows, dtype=np.int64), device=self.device)
+        return self._assemble(self._positions(idx))
+
+    def _positions(self, input_rows):
+        """Device positions of the particles with the given input indices."""
+        t = self.torch
+        inv = t.empty_like(self.pid)
+        inv[self.pid] = t.arange(self.pid.numel(), dtype=t.int64, device=self.device)
+        return inv[input_rows]
+
+    def global_ids(self):
+        """Global_ID column (15) of the particle matrix, input order."""
+        return self._meta[:, 3].astype(np.int64)
 
     def set_local_bunch(self, M):
         """Replace the device state by an (N, 16) matrix."""
@@ -216,7 +251,24 @@ class FFAG_Bunch:
 
     @property
     def Step_SurviveFlag(self):
-        return (self.ordered("flags").cpu().numpy() & _lib.F_ALIVE) != 0
+        """Survive flags (column 8) as they were before the step, when
+        ``UpdatePreSteps`` recorded them; otherwise the current flags."""
+        fl = self._pre_flags if self._pre_valid else self.flags
+        out = self.torch.empty_like(fl)
+        out[self.pid] = fl
+        return (out.cpu().numpy() & _lib.F_ALIVE) != 0
+
+    @property
+    def PreStep_Fi(self):
+        if not self._pre_valid:
+            raise RuntimeError("UpdatePreSteps() has not recorded a pre-step state")
+        out = self.torch.empty_like(self._pre_fi)
+        out[self.pid] = self._pre_fi
+        return out.cpu().numpy()
+
+    @property
+    def PostStep_Fi(self):
+        return self.ordered("fi").cpu().numpy()
 
     # ----------------------------------------------------------- bookkeeping
     def injection_range(self, t_threshold, t_step):
@@ -279,11 +331,67 @@ class FFAG_Bunch:
         self.LostNum = self.LostNum_Local
 
     def UpdatePreSteps(self):
-        # The device kernels read the survive flag directly; nothing to copy.
+        """Record the azimuth and flags before a step on the device
+        (PreStep_Fi and Step_SurviveFlag, src/FFAG_Bunch.py:395-401 and
+        src/FFAG_MathTools.py:2179-2201).  The push kernels read the flags
+        directly, so the tracker calls this only before the steps that a
+        detector examines."""
+        t = self.torch
+        n = self.TotalParticleNum_Local
+        if self._pre_fi is None or self._pre_fi.numel() != n:
+            self._pre_fi = t.empty(n, dtype=t.float64, device=self.device)
+            self._pre_flags = t.empty(n, dtype=t.uint8, device=self.device)
+        self._pre_fi.copy_(self.fi)
+        self._pre_flags.copy_(self.flags)
+        self._pre_valid = True
         self._host_cache = None
 
     def UpdatePostSteps(self):
         self._host_cache = None
+
+    def detect_crossing(self, target, input_rows=None, ctx=None):
+        """detect_azimuth_crossing (src/FFAG_MathTools.py:1633-1695) between
+        the state recorded by ``UpdatePreSteps`` and the current one, on the
+        GPU (ffa_detect_crossing).
+
+        ``target`` is the azimuth in [0, 2 pi] after the reference's
+        ``% (2 pi)``.  Without ``input_rows``: returns the crossing particles'
+        input indices in increasing order and their device positions.  With
+        ``input_rows`` (the detect_azimuth_crossing_portion form,
+        src/FFAG_MathTools.py:1697-1757): returns a boolean mask over them."""
+        t = self.torch
+        if not self._pre_valid:
+            raise RuntimeError("UpdatePreSteps() must run before the step a detector examines")
+        if ctx is None:
+            ctx = self._own_ctx()
+        if input_rows is None:
+            pos = None
+            fi_pre, fl_pre, fi_post = self._pre_fi, self._pre_flags, self.fi
+        else:
+            rows = t.as_tensor(np.asarray(input_rows, dtype=np.int64), device=self.device)
+            pos = self._positions(rows)
+            fi_pre, fl_pre, fi_post = self._pre_fi[pos], self._pre_flags[pos], self.fi[pos]
+        n = fi_pre.numel()
+        if not target < 2.0 * np.pi:
+            # float rounding of x % (2 pi) can give 2 pi itself: no azimuth
+            # reduced to [0, 2 pi) reaches it, so nothing crosses
+            hit = t.empty(0, dtype=t.int64, device=self.device)
+        else:
+            if self._det_idx is None or self._det_idx.numel() < max(n, 1):
+                self._det_idx = t.empty(max(n, 1), dtype=t.int64, device=self.device)
+                self._det_count = t.zeros(1, dtype=t.int64, device=self.device)
+            ctx.check(ctx.lib.ffa_detect_crossing(
+                ctx.h, _lib._vp(fi_pre.data_ptr()), _lib._vp(fl_pre.data_ptr()), _lib._vp(fi_post.data_ptr()),
+                n, float(target), _lib._vp(self._det_idx.data_ptr()), _lib._vp(self._det_count.data_ptr())),
+                "ffa_detect_crossing")
+            hit = self._det_idx[:int(self._det_count.item())]
+        if input_rows is not None:
+            mask = np.zeros(n, dtype=bool)
+            mask[hit.cpu().numpy()] = True
+            return mask
+        rows = self.pid[hit]
+        order = t.argsort(rows)
+        return rows[order].cpu().numpy(), hit[order]
 
     # ---------------------------------------------------------------- SC data
     def seo_device(self):

@@ -26,20 +26,7 @@ import numpy as np
 
 from . import _lib
 from .FFAG_Field import build_machine
-
-
-class _Dumps:
-    """Detector collection used when the caller passes none."""
-
-    def __init__(self):
-        self.dumps = []
-
-    def add_dump(self, d):
-        self.dumps.append(d)
-
-    def check_and_dump(self, t_pre, t_post, bunch, data_type="Boris"):
-        for d in self.dumps:
-            d.check_and_dump(t_pre, t_post, bunch, data_type)
+from .FFAG_dump import Dumps
 
 
 def FunctionForAccelerationBunch_dt(*args, **kwargs):
@@ -119,7 +106,9 @@ class FFAG_RungeKutta:
         enable_sc = bool(LocalParameters.get("enable_SC", False))
         sc_type = LocalParameters.get("SC_type", 1)
         sc_step = int(LocalParameters.get("SC_step", 1))
-        dumps = LocalParameters.get("step_dumps") or _Dumps()
+        dumps = LocalParameters.get("step_dumps")
+        if dumps is None:
+            dumps = Dumps()
         precision = LocalParameters.get("precision", BunchObj.precision)
         if enable_sc and sc_type != 1:
             raise TypeError("only SC_type=1 (2.5D slice-by-slice) is implemented on the GPU path")
@@ -151,6 +140,7 @@ class FFAG_RungeKutta:
         lib = ctx.lib
         verbose = LocalParameters.get("verbose", True)
         has_dumps = len(getattr(dumps, "dumps", [1])) > 0
+        needs_step = getattr(dumps, "needs_step", None)
         # FP32 tier: reorder particles by azimuth every sort_every steps (map
         # and SC-grid locality); the FP64 path keeps the input order
         sort_every = int(LocalParameters.get("sort_every", 1000 if precision == "fp32" else 0))
@@ -162,16 +152,27 @@ class FFAG_RungeKutta:
         while steps_i < max_steps:
             # one block: the steps up to the next SC solve / 1000-step check /
             # end; a block is one fused launch (FP32 tier) or one C-side
-            # sequence of per-step launches (FP64 path).  Detectors need the
-            # state after every step, so they force one-step blocks.
+            # sequence of per-step launches (FP64 path).  A step that a
+            # detector examines runs as a block of its own, with the pre-step
+            # state recorded; the steps before it form the previous block.
             nb = 1000 - steps_i % 1000
             if enable_sc:
                 nb = min(nb, sc_step - steps_i % sc_step)
             if sort_every > 0:
                 nb = min(nb, sort_every - steps_i % sort_every)
             nb = min(nb, max_steps - steps_i, max(1, min(max_block, _MAX_BLOCK)))
+            examine = False
             if has_dumps:
-                nb = 1
+                j = 0
+                if needs_step is not None:
+                    tt = t_pre
+                    while j < nb and not needs_step(tt, tt + step_t):
+                        tt = tt + step_t
+                        j += 1
+                if j == 0:
+                    nb, examine = 1, True
+                else:
+                    nb = j
             b0, e0 = BunchObj.injection_range(t_pre, step_t)
             if e0 > b0:
                 ctx.check(lib.ffa_inject(ctx.h, BunchObj.particles(), b0, e0), "ffa_inject")
@@ -205,6 +206,8 @@ class FFAG_RungeKutta:
                         st.rk_rate[q] = fq * np.pi * 2
                         st.rk_v0[q] = vq
                 tt = tt + step_t
+            if examine:
+                BunchObj.UpdatePreSteps()
             if fused:
                 ctx.check(lib.ffa_push_boris_fused_f32(ctx.h, BunchObj.particles(), steps, inj, nb),
                           "ffa_push_boris_fused_f32")
@@ -213,7 +216,7 @@ class FFAG_RungeKutta:
                                                  1 if solver == "rk4" else 0), "ffa_push_f64_multi")
             BunchObj.last_sc_s = steps[nb - 1].sc_s
             BunchObj.UpdatePostSteps()
-            if has_dumps:
+            if examine:
                 dumps.check_and_dump(t_pre, tt, BunchObj, data_type="Boris" if solver == "boris" else "RK4")
             t_pre = tt
             first_step = steps_i
