@@ -111,6 +111,8 @@ typedef struct {
     int32_t first;         /* first solve of the run (steps_i == 0) */
     double s_last;         /* extrapolation factor of the step before this solve */
     int32_t bits;          /* fixed-point fractional bits per CIC axis, 1..20 */
+    int32_t cell_counts;   /* 1: also store the particles per CIC base cell in the
+                              cell_count region (a checking aid; 0 leaves it unwritten) */
     /* SEO tables, device, row-major (n_ek, n_fi) */
     const double *seo_r, *seo_pr;
     const double *seo_ek_axis, *seo_perimeter;   /* n_ek each */
@@ -139,7 +141,7 @@ typedef struct {
     size_t cell_count;  /* int32 (ns, nz, nx) particles per CIC base cell */
     size_t cbuf;        /* complex64 (ceil(ns/2), nz, nx) packed slice pairs */
     size_t phi;         /* float (ns, nz, nx) potential */
-    size_t ex, ez;      /* float (ns, nz, nx) fields */
+    size_t exz;         /* float2 (ns, nz, nx): (Ex, Ez) of each cell, interleaved */
     size_t px;          /* double n: x = r - r0 */
     size_t ppr0;        /* double n: pr0 */
     size_t pslice;      /* int32 n: slice index */
@@ -151,6 +153,7 @@ typedef struct {
     size_t cursor;      /* int32 ns: fill cursors */
     size_t records_by_row;  /* uint64 n: records of each slice ordered by CIC row */
     size_t row_start;   /* int32 ns * (nz + 1): record offset of each (slice, row) */
+    size_t tables;      /* FFT twiddles (4 x 1024 complex64) and the Green's function (nx * nz float32) */
 } ffa_sc_layout;
 
 #define FFA_SC_BLOCKS 1024
@@ -208,8 +211,12 @@ int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_s
  * the endpoint roll of src/FFAG_track.py:356-382: survivor coordinates,
  * covering arc, SEO lookup, slicing and moments, fixed-point CIC deposit,
  * periodic FFT Poisson solve per slice, -grad(phi)/gamma^2, bilinear gather
- * with the pr0 projection, then Pre/PrePre endpoint fields and masks rolled
- * in place.  workspace: device block laid out by ffa_sc_workspace_layout. */
+ * with the pr0 projection, then Pre/PrePre endpoint fields and masks rolled.
+ * The roll swaps buffer roles: the new pre-step field is written into the
+ * memory p->esc_prepre points to, and the memory p->esc_pre points to (left
+ * untouched) holds the new prepre field, so the caller swaps its esc_pre and
+ * esc_prepre pointers after the call.  workspace: device block laid out by
+ * ffa_sc_workspace_layout. */
 int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace);
 
 /* Stop-condition reduction of src/FFAG_track.py:559-577: out (device, 2

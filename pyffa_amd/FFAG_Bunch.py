@@ -105,6 +105,8 @@ class FFAG_Bunch:
         self._sc_ws = None
         self._sc_layout = None
         self._host_cache = None
+        # store per-cell CIC base counts at each solve (a checking aid)
+        self.sc_cell_counts = False
 
     # ------------------------------------------------------------------ state
     def _upload(self, M):
@@ -422,6 +424,7 @@ class FFAG_Bunch:
         sp.first = 1 if first else 0
         sp.s_last = float(s_last)
         sp.bits = fixed_point_bits(max(self.TotalParticleNum_Local, 1))
+        sp.cell_counts = 1 if self.sc_cell_counts else 0
         sp.seo_r, sp.seo_pr = seo["r"].data_ptr(), seo["pr"].data_ptr()
         sp.seo_ek_axis, sp.seo_perimeter = seo["ek"].data_ptr(), seo["per"].data_ptr()
         sp.n_ek, sp.n_fi = int(self.SEO_r_matrix.shape[0]), int(self.SEO_r_matrix.shape[1])
@@ -434,12 +437,15 @@ class FFAG_Bunch:
         return sp
 
     def sc_view(self, name):
-        """Typed device view of one workspace region (after a solve)."""
+        """Typed device view of one workspace region (after a solve); "ex"
+        and "ez" are the components of the interleaved field region."""
         t = self.torch
+        if name in ("ex", "ez"):
+            return self.sc_view("exz")[..., 0 if name == "ex" else 1]
         ws, L = self.sc_workspace()
         ns, nz, nx, n = self.f_grid_size, self.z_grid_size, self.x_grid_size, self.TotalParticleNum_Local
         spec = {"grid": (t.int64, (ns, nz, nx)), "cell_count": (t.int32, (ns, nz, nx)),
-                "phi": (t.float32, (ns, nz, nx)), "ex": (t.float32, (ns, nz, nx)), "ez": (t.float32, (ns, nz, nx)),
+                "phi": (t.float32, (ns, nz, nx)), "exz": (t.float32, (ns, nz, nx, 2)),
                 "px": (t.float64, (n,)), "ppr0": (t.float64, (n,)), "pslice": (t.int32, (n,)),
                 "slice_count": (t.int32, (ns,)), "result": (t.float64, (32,))}[name]
         off = getattr(L, name)
@@ -491,6 +497,9 @@ class FFAG_Bunch:
         ws, _ = self.sc_workspace()
         sp = self.sc_params(first=first, s_last=s_last)
         ctx.check(ctx.lib.ffa_sc_solve(ctx.h, self.particles(), sp, ws.data_ptr()), "ffa_sc_solve")
+        # the solve writes the new pre-step field into the prepre buffer and
+        # leaves the old pre-step field, now the prepre one, where it was
+        self.esc_pre, self.esc_prepre = self.esc_prepre, self.esc_pre
         self._host_cache = None
 
 

@@ -63,7 +63,7 @@ class FfaStep(ctypes.Structure):
 
 class FfaScParams(ctypes.Structure):
     _fields_ = [("ns", _i32), ("nx", _i32), ("nz", _i32), ("sigma_ratio", _d), ("q_macro", _d),
-                ("rk", _i32), ("first", _i32), ("s_last", _d), ("bits", _i32),
+                ("rk", _i32), ("first", _i32), ("s_last", _d), ("bits", _i32), ("cell_counts", _i32),
                 ("seo_r", _vp), ("seo_pr", _vp), ("seo_ek_axis", _vp), ("seo_perimeter", _vp),
                 ("n_ek", _i32), ("n_fi", _i32), ("ek0", _d), ("dek", _d), ("fi0", _d), ("dfi", _d),
                 ("E0_MeV", _d), ("c", _d)]
@@ -79,8 +79,9 @@ class FfaScResult(ctypes.Structure):
 
 class FfaScLayout(ctypes.Structure):
     _fields_ = [(name, ctypes.c_size_t) for name in (
-        "total", "grid", "cell_count", "cbuf", "phi", "ex", "ez", "px", "ppr0", "pslice",
-        "slice_count", "partials", "result", "records", "slice_start", "cursor", "records_by_row", "row_start")]
+        "total", "grid", "cell_count", "cbuf", "phi", "exz", "px", "ppr0", "pslice",
+        "slice_count", "partials", "result", "records", "slice_start", "cursor", "records_by_row", "row_start",
+        "tables")]
 
 
 MLP_NFREQ = 32

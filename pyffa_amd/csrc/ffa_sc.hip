@@ -354,7 +354,8 @@ __global__ __launch_bounds__(ROW_BLOCK) void k_sc_rowsort(ffa_sc_params sp, cons
 // One workgroup per (slice, band of bz rows): zero an LDS tile, add the
 // corners of the records of rows z0 - 1 .. z1 - 1 that fall in the band (a
 // contiguous range after k_sc_rowsort), store the band.  Dynamic LDS:
-// bz * nx int64 sums, then bz * nx int32 base-cell counts.
+// bz * nx int64 sums, then (when sp.cell_counts) bz * nx int32 base-cell
+// counts.
 __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, const ffa_sc_result *res,
                                                            const int32_t *row_start, const uint64_t *rec,
                                                            unsigned long long *grid, int32_t *cell, int bz) {
@@ -364,9 +365,12 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, co
     const int s = blockIdx.x / nbands, band = blockIdx.x - s * nbands;
     const int z0 = band * bz, z1 = z0 + bz;
     const int ncell = bz * nx;
+    const bool counts = sp.cell_counts != 0;
     unsigned long long *acc = lds;
     int32_t *cnt = reinterpret_cast<int32_t *>(lds + ncell);
-    for (int k = threadIdx.x; k < ncell; k += BLOCK) { acc[k] = 0ull; cnt[k] = 0; }
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) acc[k] = 0ull;
+    if (counts)
+        for (int k = threadIdx.x; k < ncell; k += BLOCK) cnt[k] = 0;
     __syncthreads();
     if (res->bad_grid == 0.0) {
         const uint64_t one = (uint64_t)1 << sp.bits;
@@ -380,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, co
                 const int o = (iz - z0) * nx + ix;
                 atomicAdd(&acc[o], (one - a) * (one - b));
                 atomicAdd(&acc[o + 1], a * (one - b));
-                atomicAdd(&cnt[o], 1);
+                if (counts) atomicAdd(&cnt[o], 1);
             }
             if (iz + 1 >= z0 && iz + 1 < z1) {
                 const int o = (iz + 1 - z0) * nx + ix;
@@ -391,10 +395,9 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, co
     }
     __syncthreads();
     const size_t base = (size_t)s * nz * nx + (size_t)z0 * nx;
-    for (int k = threadIdx.x; k < ncell; k += BLOCK) {
-        grid[base + k] = acc[k];
-        cell[base + k] = cnt[k];
-    }
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) grid[base + k] = acc[k];
+    if (counts)
+        for (int k = threadIdx.x; k < ncell; k += BLOCK) cell[base + k] = cnt[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -407,14 +410,42 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// w^k = exp(-+ 2 pi i k / N) for k < N (the radix-4 stages use exponents up
-// to 3N/4)
-__device__ __forceinline__ void make_twiddles(float2 *tw, int N, bool inv) {
-    for (int k = threadIdx.x; k < N; k += BLOCK) {
-        double s, c;
-        sincospi(-2.0 * (double)k / (double)N, &s, &c);
-        tw[k] = make_float2((float)c, (float)(inv ? -s : s));
+// Per-solve tables (tables region): twiddles w^k for k < N of the x and z
+// transforms, forward and inverse, at [0..4) * 1024, then the periodic
+// Green's function (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760)
+// in the reference's float32 arithmetic, transposed to [ci][j] so a column
+// group reads one contiguous run.
+constexpr int TW_STRIDE = 1024;
+
+__global__ __launch_bounds__(BLOCK) void k_sc_tables(float2 *tw, float *green, ffa_sc_params sp,
+                                                    const ffa_sc_result *res) {
+    const int nx = sp.nx, nz = sp.nz;
+    const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (t < 4 * TW_STRIDE) {
+        const int which = (int)(t / TW_STRIDE), k = (int)(t - which * TW_STRIDE);
+        const int N = which < 2 ? nx : nz;
+        if (k < N) {
+            double s, c;
+            sincospi(-2.0 * (double)k / (double)N, &s, &c);
+            tw[t] = make_float2((float)c, (float)((which & 1) ? -s : s));
+        }
+        return;
     }
+    const int64_t g = t - 4 * TW_STRIDE;
+    if (g >= (int64_t)nx * nz) return;
+    const int ci = (int)(g / nz), j = (int)(g - (int64_t)ci * nz);
+    const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
+    const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
+    const int idx = ci <= nx / 2 ? ci : ci - nx;
+    const int idz = j <= nz / 2 ? j : j - nz;
+    const float kx = (float)((double)dkx * idx), kz = (float)((double)dkz * idz);
+    const float kz2 = kz * kz;
+    const float k2 = kx * kx + kz2;
+    green[g] = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
+}
+
+__device__ __forceinline__ void load_twiddles(float2 *tw, const float2 *tab, int N) {
+    for (int k = threadIdx.x; k < N; k += BLOCK) tw[k] = tab[k];
 }
 
 // In-LDS Stockham FFT of pts / N lines of N points: radix-4 stages
@@ -470,13 +501,15 @@ __device__ float2 *lds_fft4(float2 *x, float2 *y, const float2 *tw, int N, int l
 
 // rows: forward FFT along x of the packed slice pair; grid -> cbuf.
 __global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(const int64_t *grid, float2 *cbuf, ffa_sc_params sp,
-                                                       const ffa_sc_result *res, int lognx, int lognz) {
-    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS];
+                                                       const ffa_sc_result *res, const float2 *twt, int lognx,
+                                                       int lognz) {
+    extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
+    float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
     const int lines = FFT_PTS / nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
     const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
-    make_twiddles(tw, nx, false);
+    load_twiddles(tw, twt, nx);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
         const int l = e >> lognx, i = e & (nx - 1);
         const int64_t row = row0 + l;
@@ -499,19 +532,23 @@ constexpr int COL_PTS = 1024;
 
 __host__ __device__ inline int col_width(int nx, int nz) { return COL_PTS / nz < nx ? COL_PTS / nz : nx; }
 
-__global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const ffa_sc_result *res,
-                                                         int lognz) {
-    __shared__ float2 b0[COL_PTS + 32], b1[COL_PTS + 32], tw[1024], twi[1024];
+__global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const float2 *twt,
+                                                         const float *green, int lognz) {
+    // dynamic LDS: b0, b1 (COL_PTS + 32 each, padded columns), tw[nz], twi[nz]
+    extern __shared__ float2 lds2[];
+    float2 *b0 = lds2, *b1 = lds2 + (COL_PTS + 32), *tw = lds2 + 2 * (COL_PTS + 32), *twi = tw + sp.nz;
     const int nx = sp.nx, nz = sp.nz;
     const int W = col_width(nx, nz);                  // columns per block
     const int PTS = W * nz;
     const int LS = nz + 1;                            // padded column stride in LDS
     const int cpp = nx / W;                           // column groups per plane
-    const int64_t plane = blockIdx.x / cpp;
-    const int c0 = (blockIdx.x - plane * cpp) * W;
+    // neighbouring column groups share cache lines: keep them on one XCD
+    const unsigned blk = xcd_block();
+    const int64_t plane = blk / cpp;
+    const int c0 = (blk - plane * cpp) * W;
     float2 *base = cbuf + plane * (int64_t)nz * nx;
-    make_twiddles(tw, nz, false);
-    make_twiddles(twi, nz, true);
+    load_twiddles(tw, twt + 2 * TW_STRIDE, nz);
+    load_twiddles(twi, twt + 3 * TW_STRIDE, nz);
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
         b0[c * LS + j] = base[(int64_t)j * nx + c0 + c];
@@ -519,18 +556,12 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
     __syncthreads();
     float2 *o = lds_fft4(b0, b1, tw, nz, lognz, false, PTS, LS);
     float2 *other = (o == b0) ? b1 : b0;
-    // Green's function in the reference's float32 arithmetic
-    const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
-    const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
+    // Green's function of the block's columns: one contiguous run of the
+    // transposed table
+    const float *gcol = green + (int64_t)c0 * nz;
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int c = e / nz, j = e - c * nz;
-        const int ci = c0 + c;
-        const int idx = ci <= nx / 2 ? ci : ci - nx;
-        const int idz = j <= nz / 2 ? j : j - nz;
-        const float kx = (float)((double)dkx * idx), kz = (float)((double)dkz * idz);
-        const float kz2 = kz * kz;
-        const float k2 = kx * kx + kz2;
-        const float g = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
+        const float g = gcol[e];
         float2 &v = o[c * LS + j];
         v = make_float2(v.x * g, v.y * g);
     }
@@ -543,13 +574,14 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
 }
 
 // rows: inverse FFT along x, normalise by 1/(nx nz), unpack the slice pair.
-__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp, int lognx,
-                                                       int lognz) {
-    __shared__ float2 b0[FFT_PTS], b1[FFT_PTS], tw[FFT_PTS];
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp,
+                                                       const float2 *twt, int lognx, int lognz) {
+    extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
+    float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
     const int lines = FFT_PTS / nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;
-    make_twiddles(tw, nx, true);
+    load_twiddles(tw, twt + TW_STRIDE, nx);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
     __syncthreads();
     const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, true);
@@ -565,49 +597,75 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, floa
 }
 
 // compute_field_from_potential_numba_SBS (src/FFAG_SC_new.py:672-728):
-// float32 differences, float64 scaling, float32 store.
-__global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float *Ex, float *Ez, ffa_sc_params sp,
-                                                   const ffa_sc_result *res, int lognx, int lognz) {
+// float32 differences, float64 scaling, float32 store.  A thread computes
+// FIELD_ROWS consecutive rows of one column (its loads are independent, so
+// their latencies overlap; a wave covers 64 consecutive columns) and stores
+// (Ex, Ez) interleaved, so the gather reads both with one access per corner.
+constexpr int FIELD_ROWS = 4;
+
+__global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *exz, ffa_sc_params sp,
+                                                   const ffa_sc_result *res, int lognx) {
     const int nx = sp.nx, nz = sp.nz;
-    const int64_t total = (int64_t)sp.ns * nz * nx;
-    const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    const int nb = nz / FIELD_ROWS;
+    const int64_t total = (int64_t)sp.ns * nb * nx;
+    // the stencil rows j0 - 1 and j0 + FIELD_ROWS belong to neighbouring
+    // bands: keep neighbours on one XCD so each row is fetched into one L2
+    const int64_t t = xcd_block() * (int64_t)BLOCK + threadIdx.x;
     if (t >= total) return;
-    const int64_t k = t >> (lognx + lognz);                 // nx, nz are powers of two
-    const int rem = (int)(t & (((int64_t)nz << lognx) - 1));
-    const int j = rem >> lognx, i = rem & (nx - 1);
+    const int i = (int)(t & (nx - 1));
+    const int64_t r = t >> lognx;
+    const int band = (int)(r % nb);
+    const int64_t k = r / nb;
+    const int j0 = band * FIELD_ROWS;
+    const int64_t base = k * (int64_t)nz * nx;
     const double dr = res->dx, dz = res->dz, bl = res->bunch_length;
-    if (res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0) { Ex[t] = 0.0f; Ez[t] = 0.0f; return; }
+    if (res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0) {
+        for (int c = 0; c < FIELD_ROWS; ++c) exz[base + (int64_t)(j0 + c) * nx + i] = make_float2(0.0f, 0.0f);
+        return;
+    }
     const double gamma = res->mean_gamma_m1 + 1.0;
     const double ig2 = 1.0 / (gamma * gamma);
     const double i2r = 0.5 / dr, i2z = 0.5 / dz;
-    const float *P = phi + k * (int64_t)nz * nx;
-#define PH(jj, ii) P[(jj) * nx + (ii)]
-    double ex, ez;
-    if (i == 0 || i == nx - 1) {
-        const int jp = j + 1 < nz - 1 ? j + 1 : nz - 1, jm = j - 1 > 0 ? j - 1 : 0;
-        ex = i == 0 ? -(double)(PH(j, 1) - PH(j, 0)) / dr * ig2 : -(double)(PH(j, nx - 1) - PH(j, nx - 2)) / dr * ig2;
-        ez = -(double)(PH(jp, i) - PH(jm, i)) / dz * ig2;
-    } else if (j == 0) {
-        ex = -(double)(PH(0, i + 1) - PH(0, i - 1)) * i2r * ig2;
-        ez = -(double)(PH(1, i) - PH(0, i)) / dz * ig2;
-    } else if (j == nz - 1) {
-        ex = -(double)(PH(nz - 1, i + 1) - PH(nz - 1, i - 1)) * i2r * ig2;
-        ez = -(double)(PH(nz - 1, i) - PH(nz - 2, i)) / dz * ig2;
-    } else {
-        ex = -(double)(PH(j, i + 1) - PH(j, i - 1)) * i2r * ig2;
-        ez = -(double)(PH(j + 1, i) - PH(j - 1, i)) * i2z * ig2;
+    const float *P = phi + base;
+    // col[c] = phi(clamp(j0 - 1 + c), i); lf/rt = phi(j, i -+ 1) clamped to the row
+    float col[FIELD_ROWS + 2], lf[FIELD_ROWS], rt[FIELD_ROWS];
+    const int il = i > 0 ? i - 1 : 0, ir = i < nx - 1 ? i + 1 : nx - 1;
+    for (int c = 0; c < FIELD_ROWS + 2; ++c) {
+        int j = j0 - 1 + c;
+        j = j < 0 ? 0 : (j > nz - 1 ? nz - 1 : j);
+        col[c] = P[(int64_t)j * nx + i];
     }
-#undef PH
-    Ex[t] = (float)ex;
-    Ez[t] = (float)ez;
+    for (int c = 0; c < FIELD_ROWS; ++c) {
+        lf[c] = P[(int64_t)(j0 + c) * nx + il];
+        rt[c] = P[(int64_t)(j0 + c) * nx + ir];
+    }
+    for (int c = 0; c < FIELD_ROWS; ++c) {
+        const int j = j0 + c;
+        const float pc = col[c + 1], pu = col[c + 2], pd = col[c];
+        double ex, ez;
+        if (i == 0 || i == nx - 1) {              // one-sided in x; z neighbours clamped
+            ex = i == 0 ? -(double)(rt[c] - pc) / dr * ig2 : -(double)(pc - lf[c]) / dr * ig2;
+            ez = -(double)(pu - pd) / dz * ig2;
+        } else if (j == 0) {
+            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
+            ez = -(double)(pu - pc) / dz * ig2;
+        } else if (j == nz - 1) {
+            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
+            ez = -(double)(pc - pd) / dz * ig2;
+        } else {
+            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
+            ez = -(double)(pu - pd) * i2z * ig2;
+        }
+        exz[base + (int64_t)j * nx + i] = make_float2((float)ex, (float)ez);
+    }
 }
 
 // interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
 // fused with the endpoint roll of src/FFAG_track.py:356-382.
 template <typename ET>
 __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
-                                                         const float *Exc, const float *Ezc, const double *px,
-                                                         const double *ppr0, const int32_t *pslice) {
+                                                         const float2 *exz, const double *px, const double *ppr0,
+                                                         const int32_t *pslice) {
     const int64_t i = xcd_block() * (int64_t)BLOCK + threadIdx.x;
     if (i >= p.n) return;
     const int64_t n = p.n;
@@ -615,15 +673,19 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
     ET *epp = reinterpret_cast<ET *>(p.esc_prepre);
     const uint8_t fl = p.flags[i];
     const bool alive = fl & FFA_F_ALIVE;
-    const double o_r = (double)ep[i], o_z = (double)ep[n + i], o_f = (double)ep[2 * n + i];
+    const bool good = res->bad_grid == 0.0;
+    // the old pre-step field is read only where it survives the solve
+    double o_r = 0.0, o_z = 0.0, o_f = 0.0;
+    if (!(alive && good)) { o_r = (double)ep[i]; o_z = (double)ep[n + i]; o_f = (double)ep[2 * n + i]; }
     double nr = o_r, nz_ = o_z, nf = o_f;
     if (alive) {
-        if (res->bad_grid == 0.0) {
+        if (good) {
             const int nx = sp.nx, nz = sp.nz;
             int s = pslice[i];
             if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-            const float *Ex = Exc + (size_t)s * nz * nx, *Ez = Ezc + (size_t)s * nz * nx;
+            const float2 *E = exz + (size_t)s * nz * nx;
             const double t = (px[i] - res->xmin) * (1.0 / res->dx), u = (p.z[i] - res->zmin) * (1.0 / res->dz);
+            const double pr = ppr0[i];
             const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
             int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cu);
             if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
@@ -631,9 +693,9 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             const double al = t - (double)i0, be = u - (double)j0;
             const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
             const int64_t b = j0 * nx + i0;
-            const double er = w00 * Ex[b] + w10 * Ex[b + 1] + w01 * Ex[b + nx] + w11 * Ex[b + nx + 1];
-            const double ez = w00 * Ez[b] + w10 * Ez[b + 1] + w01 * Ez[b + nx] + w11 * Ez[b + nx + 1];
-            const double pr = ppr0[i];
+            const float2 a00 = E[b], a10 = E[b + 1], a01 = E[b + nx], a11 = E[b + nx + 1];
+            const double er = w00 * a00.x + w10 * a10.x + w01 * a01.x + w11 * a11.x;
+            const double ez = w00 * a00.y + w10 * a10.y + w01 * a01.y + w11 * a11.y;
             const double ip = 1.0 / sqrt(1.0 + pr * pr);
             nr = er * ip;
             nz_ = ez;
@@ -646,15 +708,17 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             nf = o_f + sp.s_last * (o_f - (double)epp[2 * n + i]);
         }
     }
-    uint8_t nfl = fl;
+    // The two endpoint buffers trade roles at every solve: the new pre-step
+    // field goes into the old prepre memory (the caller then swaps its
+    // pointers) and the old pre memory, untouched, is the new prepre field.
+    // A live particle writes only its new field; a dead one keeps its old.
+    const ET a = (ET)nr, b = (ET)nz_, c = (ET)nf;
+    epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
+    uint8_t nfl;
     if (sp.first) {
-        const ET a = alive ? (ET)nr : ep[i], b = alive ? (ET)nz_ : ep[n + i], c = alive ? (ET)nf : ep[2 * n + i];
         ep[i] = a; ep[n + i] = b; ep[2 * n + i] = c;
-        epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
         nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | (alive ? FFA_F_SCPRE : 0));
     } else {
-        epp[i] = ep[i]; epp[n + i] = ep[n + i]; epp[2 * n + i] = ep[2 * n + i];
-        if (alive) { ep[i] = (ET)nr; ep[n + i] = (ET)nz_; ep[2 * n + i] = (ET)nf; }
         nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | ((fl & FFA_F_SCPRE) ? FFA_F_SCPREPRE : 0) |
                         (alive ? FFA_F_SCPRE : 0));
     }
@@ -680,8 +744,7 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     L->cell_count = o; o = al(o + 4 * C);
     L->cbuf = o; o = al(o + 8 * P);
     L->phi = o; o = al(o + 4 * C);
-    L->ex = o; o = al(o + 4 * C);
-    L->ez = o; o = al(o + 4 * C);
+    L->exz = o; o = al(o + 8 * C);
     L->px = o; o = al(o + 8 * (size_t)n);
     L->ppr0 = o; o = al(o + 8 * (size_t)n);
     L->pslice = o; o = al(o + 4 * (size_t)n);
@@ -693,6 +756,7 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     L->cursor = o; o = al(o + 4 * (size_t)ns);
     L->records_by_row = o; o = al(o + 8 * (size_t)n);
     L->row_start = o; o = al(o + 4 * (size_t)ns * ((size_t)nz + 1));
+    L->tables = o; o = al(o + 8 * 4 * (size_t)TW_STRIDE + 4 * (size_t)nz * nx);
     L->total = o;
     return 0;
 }
@@ -704,8 +768,7 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     int32_t *cell = reinterpret_cast<int32_t *>(w + L->cell_count);
     float2 *cbuf = reinterpret_cast<float2 *>(w + L->cbuf);
     float *phi = reinterpret_cast<float *>(w + L->phi);
-    float *ex = reinterpret_cast<float *>(w + L->ex);
-    float *ez = reinterpret_cast<float *>(w + L->ez);
+    float2 *exz = reinterpret_cast<float2 *>(w + L->exz);
     double *px = reinterpret_cast<double *>(w + L->px);
     double *ppr0 = reinterpret_cast<double *>(w + L->ppr0);
     int32_t *pslice = reinterpret_cast<int32_t *>(w + L->pslice);
@@ -726,7 +789,7 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart, cursor);
     hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, cursor, rec);
     const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
-    const size_t tile_lds = (size_t)bz * sp->nx * 12;
+    const size_t tile_lds = (size_t)bz * sp->nx * (sp->cell_counts ? 12 : 8);
     uint64_t *rec2 = reinterpret_cast<uint64_t *>(w + L->records_by_row);
     int32_t *rstart = reinterpret_cast<int32_t *>(w + L->row_start);
     hipLaunchKernelGGL(k_sc_rowsort, dim3((unsigned)sp->ns), dim3(ROW_BLOCK), 0, st, *sp, res, sstart, rec, rec2,
@@ -734,22 +797,29 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st, *sp,
                        res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
     const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
+    float2 *twt = reinterpret_cast<float2 *>(w + L->tables);
+    float *green = reinterpret_cast<float *>(w + L->tables + 8 * 4 * (size_t)TW_STRIDE);
+    hipLaunchKernelGGL(k_sc_tables, dim3(gridn(4 * TW_STRIDE + (int64_t)sp->nx * sp->nz)), dim3(BLOCK), 0, st, twt, green,
+                       *sp, res);
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
-    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, grid, cbuf, *sp,
-                       res, lognx, lognz);
-    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK), 0, st,
-                       cbuf, *sp, res, lognz);
-    hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), 0, st, cbuf, phi, *sp,
-                       lognx, lognz);
-    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)C)), dim3(BLOCK), 0, st, phi, ex, ez, *sp, res, lognx, lognz);
+    const size_t rows_lds = sizeof(float2) * (2 * FFT_PTS + sp->nx);
+    const size_t cols_lds = sizeof(float2) * (2 * (COL_PTS + 32) + 2 * sp->nz);
+    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, grid, cbuf,
+                       *sp, res, twt, lognx, lognz);
+    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK),
+                       cols_lds, st, cbuf, *sp, twt, green, lognz);
+    hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf, phi,
+                       *sp, twt, lognx, lognz);
+    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)(C / FIELD_ROWS))), dim3(BLOCK), 0, st, phi, exz, *sp, res,
+                       lognx);
     if (p->n > 0) {
         if (p->esc_f32)
-            hipLaunchKernelGGL(k_sc_gather_roll<float>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, ex, ez,
-                               px, ppr0, pslice);
+            hipLaunchKernelGGL(k_sc_gather_roll<float>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, exz, px,
+                               ppr0, pslice);
         else
-            hipLaunchKernelGGL(k_sc_gather_roll<double>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, ex, ez,
-                               px, ppr0, pslice);
+            hipLaunchKernelGGL(k_sc_gather_roll<double>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, exz, px,
+                               ppr0, pslice);
     }
     return hipGetLastError();
 }

@@ -57,26 +57,70 @@ __global__ __launch_bounds__(SBLOCK) void k_sort_hist(const uint16_t *__restrict
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];      // digit-major
 }
 
-// exclusive scan of 256 * ntiles counters (digit-major), one block
-__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *__restrict__ hist, int total) {
-    __shared__ uint32_t part[1024];
-    const int per = (total + 1023) / 1024;
-    const int b = threadIdx.x * per, e = min(b + per, total);
-    uint32_t s = 0;
-    for (int k = b; k < e; ++k) s += hist[k];
-    part[threadIdx.x] = s;
+// Exclusive scan of the 256 * ntiles counters (digit-major) in three
+// coalesced passes: segment sums, a scan of the segment sums, and a scan of
+// each segment from its carry.  A segment is one tile's worth (4096).
+constexpr int SCAN_SEG = TILE;
+
+// exclusive scan across the block; *total receives the block's sum
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
+    sh[threadIdx.x] = v;
     __syncthreads();
-    for (int w = 1; w < 1024; w <<= 1) {
-        const uint32_t add = threadIdx.x >= w ? part[threadIdx.x - w] : 0u;
+    for (int w = 1; w < SBLOCK; w <<= 1) {
+        const uint32_t a = (int)threadIdx.x >= w ? sh[threadIdx.x - w] : 0u;
         __syncthreads();
-        part[threadIdx.x] += add;
+        sh[threadIdx.x] += a;
         __syncthreads();
     }
-    uint32_t acc = part[threadIdx.x] - s;
-    for (int k = b; k < e; ++k) {
-        const uint32_t v = hist[k];
-        hist[k] = acc;
-        acc += v;
+    const uint32_t incl = sh[threadIdx.x];
+    *total = sh[SBLOCK - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_scan_sums(const uint32_t *__restrict__ hist, int64_t total,
+                                                     uint32_t *__restrict__ seg_sum) {
+    __shared__ uint32_t sh[SBLOCK];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_SEG;
+    uint32_t v = 0;
+    for (int k = 0; k < PER_THREAD; ++k) {
+        const int64_t e = base + k * SBLOCK + threadIdx.x;
+        if (e < total) v += hist[e];
+    }
+    uint32_t tot;
+    block_excl_scan(v, sh, &tot);
+    if (threadIdx.x == 0) seg_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_scan_top(uint32_t *__restrict__ seg_sum, int nseg) {
+    __shared__ uint32_t sh[SBLOCK];
+    uint32_t carry = 0;
+    for (int b = 0; b < nseg; b += SBLOCK) {            // uniform trip count
+        const int k = b + (int)threadIdx.x;
+        const uint32_t v = k < nseg ? seg_sum[k] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, sh, &tot);
+        if (k < nseg) seg_sum[k] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_scan_apply(uint32_t *__restrict__ hist, int64_t total,
+                                                      const uint32_t *__restrict__ seg_off) {
+    __shared__ uint32_t sh[SBLOCK];
+    // thread t owns the PER_THREAD consecutive counters at t * PER_THREAD
+    const int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * PER_THREAD;
+    uint32_t v[PER_THREAD];
+    uint32_t s = 0;
+    for (int k = 0; k < PER_THREAD; ++k) {
+        v[k] = base + k < total ? hist[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t acc = seg_off[blockIdx.x] + block_excl_scan(s, sh, &tot);
+    for (int k = 0; k < PER_THREAD; ++k) {
+        if (base + k < total) hist[base + k] = acc;
+        acc += v[k];
     }
 }
 
@@ -164,7 +208,8 @@ int64_t sort_tiles(int64_t count) { return (count + TILE - 1) / TILE; }
 size_t sort_workspace_bytes(int64_t count) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const int64_t nt = sort_tiles(count);
-    return 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4) + al((size_t)nt * 256 * 4);
+    const int64_t nseg = (nt * 256 + SCAN_SEG - 1) / SCAN_SEG;
+    return 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4) + al((size_t)nt * 256 * 4) + al((size_t)nseg * 4);
 }
 
 hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
@@ -177,6 +222,8 @@ hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count
     uint32_t *i0 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2));
     uint32_t *i1 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + al((size_t)count * 4));
     uint32_t *hist = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4));
+    uint32_t *seg = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + al((size_t)nt * 256 * 4));
+    const int64_t total = nt * 256, nseg = (total + SCAN_SEG - 1) / SCAN_SEG;
     *order_out = i0;
     if (count == 0) return hipSuccess;
     const unsigned g = (unsigned)((count + SBLOCK - 1) / SBLOCK);
@@ -187,7 +234,9 @@ hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count
         uint32_t *ii = pass == 0 ? i0 : i1, *io = pass == 0 ? i1 : i0;
         hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, count, shift, (int)nt,
                            hist);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, c->stream, hist, (int)(256 * nt));
+        hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nseg), dim3(SBLOCK), 0, c->stream, hist, total, seg);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SBLOCK), 0, c->stream, seg, (int)nseg);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nseg), dim3(SBLOCK), 0, c->stream, hist, total, seg);
         hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, ii, count, shift,
                            (int)nt, hist, ko, io);
     }

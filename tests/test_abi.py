@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from pyffa_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,8 +58,8 @@ def test_mlp_workspace_layout():
 
 def test_workspace_layout_is_aligned_and_ordered():
     L = _lib.sc_layout(1000, 33, 64, 32)
-    offs = [L.grid, L.cell_count, L.cbuf, L.phi, L.ex, L.ez, L.px, L.ppr0, L.pslice, L.slice_count, L.partials,
-            L.result, L.records, L.slice_start, L.cursor, L.records_by_row, L.row_start]
+    offs = [L.grid, L.cell_count, L.cbuf, L.phi, L.exz, L.px, L.ppr0, L.pslice, L.slice_count, L.partials,
+            L.result, L.records, L.slice_start, L.cursor, L.records_by_row, L.row_start, L.tables]
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.total > offs[-1]
     assert L.cell_count - L.grid >= 8 * 33 * 64 * 32
@@ -80,3 +82,35 @@ def test_missing_gpu_fails_loudly():
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     with pytest.raises(_lib.FfaError):
         FFAG_Bunch(np.zeros((4, 16)))
+
+
+STRUCTS = [("FfaParticles", "ffa_particles"), ("FfaMachine", "ffa_machine"), ("FfaStep", "ffa_step"),
+           ("FfaScParams", "ffa_sc_params"), ("FfaScResult", "ffa_sc_result"), ("FfaScLayout", "ffa_sc_layout"),
+           ("FfaMlpDesc", "ffa_mlp_desc"), ("FfaAdam", "ffa_adam"), ("FfaMlpLayout", "ffa_mlp_layout")]
+
+
+def test_ctypes_mirrors_match_the_compiled_header(tmp_path):
+    """Every ctypes struct has the size and field offsets that a C compiler
+    gives the header's struct (gcc on include/ffa_hip.h)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(ROOT, "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "ffa_hip.h"', "int main(void) {"]
+    want = []
+    for py, c in STRUCTS:
+        cls = getattr(_lib, py)
+        lines.append(f'printf("%zu\\n", sizeof({c}));')
+        want.append(ctypes.sizeof(cls))
+        for f in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({c}, {f[0]}));')
+            want.append(getattr(cls, f[0]).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
