@@ -58,7 +58,7 @@ typedef struct {
  * src/FFAG_Field.py:168-198, FFAG_BField_Error src/FFAG_Field.py:965-998,
  * constants src/FFAG_ParasAndConversion.py:13-19). */
 typedef struct {
-    /* Taylor maps, device, interleaved [r_size][fi_size][max_order][3] with the
+    /* Taylor maps, device, interleaved [fi_size][r_size][max_order][3] with the
      * component order (Bz, Bfi, Br); orders 1..max_order only. */
     const void *taylor;
     int32_t taylor_f32;         /* element type of taylor: 0 double, 1 float */

@@ -110,8 +110,10 @@ class FFAG_Bfield_analytical:
         return s
 
     def device_taylor(self, dtype, device):
-        """Taylor orders 1..max_order interleaved [r][phi][order][Bz, Bfi, Br]
-        on the device (cached per dtype)."""
+        """Taylor orders 1..max_order interleaved [phi][r][order][Bz, Bfi, Br]
+        on the device (cached per dtype).  Azimuth-major: particles that
+        share an azimuth cell (neighbours after the FP32 tier's azimuth
+        ordering) read neighbouring r cells, a few cache lines per wave."""
         import torch
         key = (str(dtype), str(device))
         if key in self._dev_cache:
@@ -123,7 +125,7 @@ class FFAG_Bfield_analytical:
             for k in range(1, self.max_order + 1):
                 for st in (self.Bz_stack, self.Bfi_stack, self.Br_stack):
                     parts.append(torch.from_numpy(np.ascontiguousarray(st[k])).to(device=device, dtype=dtype))
-            t = torch.stack(parts, dim=-1).contiguous()       # (NR, NPHI, 3*max_order)
+            t = torch.stack(parts, dim=-1).permute(1, 0, 2).contiguous()    # (NPHI, NR, 3*max_order)
         self._dev_cache[key] = t
         return t
 

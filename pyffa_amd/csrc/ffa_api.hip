@@ -108,6 +108,8 @@ int ffa_set_machine(void *ctx, const ffa_machine *m) {
     if (m->taylor_f32 < 0 || m->taylor_f32 > 1) return fail(c, FFA_ERR_INVALID, "taylor_f32 must be 0 or 1");
     if (m->max_order > 0) {
         if (!m->taylor) return fail(c, FFA_ERR_INVALID, "max_order > 0 but no Taylor map");
+        if (m->taylor_f32 && (double)m->r_size * m->fi_size * 3.0 * m->max_order >= 4294967296.0)
+            return fail(c, FFA_ERR_INVALID, "FP32 Taylor map exceeds 2^32 elements (32-bit offsets)");
         if (m->r_size < 2 || m->fi_size < 2) return fail(c, FFA_ERR_INVALID, "Taylor map needs >= 2 nodes per axis");
         if (!(m->r_step > 0.0) || !(m->fi_step > 0.0)) return fail(c, FFA_ERR_INVALID, "map steps must be positive");
     }
@@ -399,11 +401,17 @@ void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {
         f.seg[j][1] = (float)m.convex[j][1];
         f.seg[j][2] = (float)(16.0 / m.convex[j][2]);
         f.seg[j][3] = (float)(m.convex[j][4] * std::fabs(m.convex[j][3]));
+        f.seg[j][4] = (float)(16.0 / m.convex[j][2] * 1.4426950408889634);     // ks / ln 2
     }
-    f.n_sp = m.n_spoly;
-    f.n_spp = m.n_sppoly;
-    for (int a = 0; a < m.n_spoly; ++a) f.sp[a] = (float)m.s_poly[a];
-    for (int a = 0; a < m.n_sppoly; ++a) f.spp[a] = (float)m.sp_poly[a];
+    // Horner from the highest degree: leading zero coefficients add exact
+    // zeros, so they are dropped (a zero spiral costs no work)
+    int a0 = 0, b0 = 0;
+    while (a0 < m.n_spoly && (float)m.s_poly[a0] == 0.0f) ++a0;
+    while (b0 < m.n_sppoly && (float)m.sp_poly[b0] == 0.0f) ++b0;
+    f.n_sp = m.n_spoly - a0;
+    f.n_spp = m.n_sppoly - b0;
+    for (int a = a0; a < m.n_spoly; ++a) f.sp[a - a0] = (float)m.s_poly[a];
+    for (int a = b0; a < m.n_sppoly; ++a) f.spp[a - b0] = (float)m.sp_poly[a];
     f.no = m.max_order;
     f.NX = m.r_size;
     f.NY = m.fi_size;

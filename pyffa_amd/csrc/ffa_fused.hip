@@ -23,6 +23,13 @@ namespace {
 constexpr int BLOCK = 256;
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// hardware square root, base-2 logarithm and exponential (about 1 ulp; the
+// FP32 tier's field error budget is far above that)
+__device__ __forceinline__ float hw_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_log2f(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
+
+typedef const __attribute__((address_space(1))) float gfloat;
 
 __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float z,
                                            float xf, float yf, float v0, double rf, float &Bz, float &Br,
@@ -39,7 +46,7 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
         B0r = k->B0;
         B0r_p = 0.0f;
     } else {
-        B0r = k->B0 * __expf(k->kv * __logf(r));
+        B0r = k->B0 * hw_exp2f(k->kv * hw_log2f(r));
         B0r_p = B0r * k->kv * inv_r;
     }
     float fs = fim - spiral;
@@ -47,8 +54,9 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
     float F = 0.0f, Fp = 0.0f;
     for (int j = 0; j < k->n_seg; ++j) {
         const float x0 = k->seg[j][0], x1 = k->seg[j][1], ks = k->seg[j][2], yc = k->seg[j][3];
-        const float s1 = rcpf(1.0f + __expf(-ks * (fs - x0)));
-        const float s2 = rcpf(1.0f + __expf(ks * (fs - x1)));
+        const float k2 = k->seg[j][4];
+        const float s1 = rcpf(1.0f + hw_exp2f(-k2 * (fs - x0)));
+        const float s2 = rcpf(1.0f + hw_exp2f(k2 * (fs - x1)));
         const float y = yc * s1 * s2;
         F += y;
         Fp += y * ks * (s2 - s1);      // ks s1 s2 ((1-s1) - (1-s2))
@@ -68,24 +76,27 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
         const float w00 = (1.0f - tx) * (1.0f - ty), w01 = (1.0f - tx) * ty, w10 = tx * (1.0f - ty), w11 = tx * ty;
         const int sj = 3 * k->no;
         const float z2 = z * z;
-        // (i, j), (i, j+1) and (i+1, j), (i+1, j+1): two contiguous runs of 2 sj floats
-        const float2 *a0 = reinterpret_cast<const float2 *>(k->T + ((int64_t)i * k->NY + j) * sj);
-        const float2 *a1 = reinterpret_cast<const float2 *>(k->T + ((int64_t)(i + 1) * k->NY + j) * sj);
+        // azimuth-major map: (i, j), (i+1, j) and (i, j+1), (i+1, j+1) are two
+        // contiguous runs of 2 sj floats; the lanes of a wave (one azimuth
+        // cell after the ordering) read neighbouring runs
+        // 32-bit offsets (the host caps the map at 2^32 elements)
+        gfloat *T = (gfloat *)(k->T);          // global address space: global_load, not flat
+        const unsigned o0 = (unsigned)(j * k->NX + i) * (unsigned)sj, o1 = o0 + (unsigned)(k->NX * sj);
+        gfloat *a0 = T + o0, *a1 = T + o1;
         float ze = z2, zo = z2 * z;
         if (k->no == 2) {
-            // 12 floats per run: order1 (Bz,Bfi,Br), order2 (...), for j and j+1
+            // 12 floats per run: order1 (Bz,Bfi,Br), order2 (...), for i and i+1
             float c0[12], c1[12];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const float2 u = a0[q], w = a1[q];
-                c0[2 * q] = u.x; c0[2 * q + 1] = u.y;
-                c1[2 * q] = w.x; c1[2 * q + 1] = w.y;
+            for (int q = 0; q < 12; ++q) {
+                c0[q] = a0[q];
+                c1[q] = a1[q];
             }
 #pragma unroll
             for (int n = 0; n < 2; ++n) {
-                const float bz = w00 * c0[3 * n] + w01 * c0[6 + 3 * n] + w10 * c1[3 * n] + w11 * c1[6 + 3 * n];
-                const float bf = w00 * c0[3 * n + 1] + w01 * c0[7 + 3 * n] + w10 * c1[3 * n + 1] + w11 * c1[7 + 3 * n];
-                const float br = w00 * c0[3 * n + 2] + w01 * c0[8 + 3 * n] + w10 * c1[3 * n + 2] + w11 * c1[8 + 3 * n];
+                const float bz = w00 * c0[3 * n] + w01 * c1[3 * n] + w10 * c0[6 + 3 * n] + w11 * c1[6 + 3 * n];
+                const float bf = w00 * c0[3 * n + 1] + w01 * c1[3 * n + 1] + w10 * c0[7 + 3 * n] + w11 * c1[7 + 3 * n];
+                const float br = w00 * c0[3 * n + 2] + w01 * c1[3 * n + 2] + w10 * c0[8 + 3 * n] + w11 * c1[8 + 3 * n];
                 Bz += bz * ze;
                 Bfi += bf * zo;
                 Br += br * zo;
@@ -93,9 +104,9 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
                 zo *= z2;
             }
         } else {
-            const float *p0 = reinterpret_cast<const float *>(a0), *p1 = reinterpret_cast<const float *>(a1);
+            gfloat *p0 = a0, *p1 = a1;
             for (int n = 0; n < k->no; ++n) {
-                const float *p00 = p0 + 3 * n, *p01 = p0 + sj + 3 * n, *p10 = p1 + 3 * n, *p11 = p1 + sj + 3 * n;
+                gfloat *p00 = p0 + 3 * n, *p10 = p0 + sj + 3 * n, *p01 = p1 + 3 * n, *p11 = p1 + sj + 3 * n;
                 Bz += (w00 * p00[0] + w01 * p01[0] + w10 * p10[0] + w11 * p11[0]) * ze;
                 Bfi += (w00 * p00[1] + w01 * p01[1] + w10 * p10[1] + w11 * p11[1]) * zo;
                 Br += (w00 * p00[2] + w01 * p01[2] + w10 * p10[2] + w11 * p11[2]) * zo;
@@ -203,10 +214,12 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
         const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
         bool use_sur = false;
         float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
+        const float xf = (float)x, yf = (float)y, zf = (float)z;
+        const float rr = hw_sqrtf(xf * xf + yf * yf);
+        const float th = atan2f(yf, xf);
         if constexpr (SUR) {
-            const float xq = (float)x, yq = (float)y, zq = (float)z;
             float a, u, c;
-            mlp::inputs(md, sqrtf(xq * xq + yq * yq), atan2f(yq, xq), zq, a, u, c);
+            mlp::inputs(md, rr, th, zf, a, u, c);
             use_sur = live && fabsf(c) <= 1.0f;
             if (__ballot(use_sur) != 0) {          // wave-uniform: all 64 lanes evaluate
                 float o[3];
@@ -223,9 +236,6 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
             const float sc = (float)s.sc_s;
             esr = e0r + sc * d_r; esz = e0z + sc * d_z; esf = e0f + sc * d_f;
         }
-        const float xf = (float)x, yf = (float)y, zf = (float)z;
-        const float rr = sqrtf(xf * xf + yf * yf);
-        const float th = atan2f(yf, xf);
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
         fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
                    SUR ? (!use_sur && md->fallback != 0) : true);
@@ -247,7 +257,10 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
         const double dt = s.dt;
         const double eps = 0.5 * dt * q_over_m;
         double mx = ux + eps * Ex, my = uy + eps * Ey, mz = uz + eps * Ezc;
-        const double ig = rsqrt(1.0 + (mx * mx + my * my + mz * mz) * inv_c2);
+        // 1 / gamma at the half step only scales the rotation vector of the
+        // FP32 field: single precision suffices (the rotation conserves |u|
+        // for any t); the final 1 / gamma, which sets the velocity, is FP64
+        const double ig = (double)__builtin_amdgcn_rsqf((float)(1.0 + (mx * mx + my * my + mz * mz) * inv_c2));
         const double tx = eps * ig * Bx, ty = eps * ig * By, tz = eps * ig * Bzc;
         const double pxp = mx + (my * tz - mz * ty), pyp = my + (mz * tx - mx * tz), pzp = mz + (mx * ty - my * tx);
         // 2 / (1 + t^2): |t| ~ omega dt / 2 is small, so the series to t^6

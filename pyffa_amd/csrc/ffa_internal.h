@@ -16,7 +16,7 @@ struct ffa_fconst {
     float P, invP;                 // sector period, 1 / period
     float B0, kv;
     int32_t n_seg, n_sp, n_spp, no;
-    float seg[FFA_MAX_SEG][4];     // x0, x1, 16/W0, y02 |y01|
+    float seg[FFA_MAX_SEG][5];     // x0, x1, ks = 16/W0, y02 |y01|, ks log2(e)
     float sp[FFA_MAX_POLY], spp[FFA_MAX_POLY];
     int32_t NX, NY;
     float r_min, inv_rstep, fi_min, inv_fistep;

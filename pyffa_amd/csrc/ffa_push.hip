@@ -57,7 +57,8 @@ __device__ __forceinline__ void order0_f64(const ffa_machine &m, double ri, doub
     Br = (B0r_p * F - B0r * Fp * spiral_p) * zi;
 }
 
-// Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) on the interleaved map,
+// Bmap_add_all_orders (src/FFAG_MathTools.py:1139-1188) on the interleaved
+// azimuth-major map [phi][r][order][Bz, Bfi, Br],
 // with the z powers of order 1 passed in: (z2, z^2, z^3) for the field,
 // (z2, zt^2, zt^3) for the coefficient form of ffa_bfield_eval mode 2.
 __device__ __forceinline__ void taylor_add_pow_f64(const ffa_machine &m, double x, double y, double z2, double ze0,
@@ -77,12 +78,13 @@ __device__ __forceinline__ void taylor_add_pow_f64(const ffa_machine &m, double 
     const double w10 = tx * (1.0 - ty), w11 = tx * ty;
     const double *T = reinterpret_cast<const double *>(m.taylor);
     const int64_t stride_j = 3 * (int64_t)no;
-    const double *a0 = T + (i * NY + j) * stride_j;          // (i, j), (i, j+1) contiguous
-    const double *a1 = T + ((i + 1) * NY + j) * stride_j;    // (i+1, j), (i+1, j+1)
+    // azimuth-major map: (i, j), (i+1, j) contiguous, then (i, j+1), (i+1, j+1)
+    const double *a0 = T + (j * NX + i) * stride_j;
+    const double *a1 = T + ((j + 1) * NX + i) * stride_j;
     double ze = ze0, zo = zo0;
     for (int n = 0; n < no; ++n) {
-        const double *p00 = a0 + 3 * n, *p01 = a0 + stride_j + 3 * n;
-        const double *p10 = a1 + 3 * n, *p11 = a1 + stride_j + 3 * n;
+        const double *p00 = a0 + 3 * n, *p10 = a0 + stride_j + 3 * n;
+        const double *p01 = a1 + 3 * n, *p11 = a1 + stride_j + 3 * n;
         const double bz = w00 * p00[0] + w01 * p01[0] + w10 * p10[0] + w11 * p11[0];
         const double bfi = w00 * p00[1] + w01 * p01[1] + w10 * p10[1] + w11 * p11[1];
         const double br = w00 * p00[2] + w01 * p01[2] + w10 * p10[2] + w11 * p11[2];
