@@ -82,6 +82,15 @@ typedef struct {
     double gap_px[FFA_MAX_GAP][4], gap_py[FFA_MAX_GAP][4];
     double gap_box[FFA_MAX_GAP][4];     /* x_min, x_max, y_min, y_max */
     double rf_shift[FFA_MAX_GAP];
+    /* spiral RF gaps (FFAG_EField_spiral, src/FFAG_Field.py:379-498):
+     * gap_model 1 replaces the fixed rectangles by the first gap whose azimuth
+     * window gap_theta[g] + [emap_phi0, emap_phi0 + (emap_nphi-1) emap_dphi]
+     * (shifted by whole turns) holds the particle, with the fields bilinear in
+     * (r, local azimuth) from three maps scaled by V0 acc_etot */
+    int32_t gap_model;                  /* 0 fixed rectangles, 1 spiral maps */
+    int32_t emap_nr, emap_nphi;
+    double emap_r0, emap_dr, emap_phi0, emap_dphi;
+    const void *emap;                   /* device double [emap_nr][emap_nphi][3]: Er, Ez / z, Ephi */
     /* aperture */
     int32_t aperture_enable;
     double aperture[4];                 /* rmin, rmax, zmin, zmax */

@@ -171,6 +171,50 @@ void ora_emap_gap(const double *M, int64_t n, double acc_Etot, double theta_n,
     }
 }
 
+/* FFAG_EField_spiral.Interpolation2D_EMap, src/FFAG_Field.py:464-498:
+ * interpolate_spiral_emap_unique_gap_kernel, shift_to_nearest_window and
+ * Bilinear_interp_2D_vect_uniform_3fields, src/FFAG_MathTools.py:939-1135.
+ * maps: (nr, nphi) row-major Er, Ez/z, Ephi.  A point outside every gap
+ * window gets zero fields and gap index 0 (the reference's -1 selects an
+ * rf_shift entry that multiplies a zero field). */
+void ora_emap_spiral(const double *M, int64_t n, const double *gap_az, int n_gap, double rf_amp, double r0,
+                     double dr, int64_t nr, double phi0, double dphi, int64_t nphi, const double *Er_map,
+                     const double *Ez_map, const double *Ephi_map, double *Er, double *Ez, double *Efi,
+                     int32_t *gap_idx) {
+    const double twopi = 2.0 * 3.141592653589793;
+    const double phi_min = phi0, phi_max = phi0 + dphi * (double)(nphi - 1);
+    const double center = 0.5 * (phi_min + phi_max);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        const double r = M[k * NCOL + 0], z = M[k * NCOL + 2], fi = M[k * NCOL + 4];
+        Er[k] = 0.0; Ez[k] = 0.0; Efi[k] = 0.0; gap_idx[k] = 0;
+        int hit = -1;
+        double loc = 0.0;
+        for (int g = 0; g < n_gap; ++g) {
+            const double ph = fi - gap_az[g];
+            const double kk = nearbyint((center - ph) / twopi);      /* np.round: half to even */
+            const double pl = ph + kk * twopi;
+            if (pl >= phi_min && pl <= phi_max) { hit = g; loc = pl; break; }
+        }
+        if (hit < 0) continue;
+        int64_t i = (int64_t)floor(clamp_for_cast((r - r0) / dr));
+        int64_t j = (int64_t)floor(clamp_for_cast((loc - phi0) / dphi));
+        if (i < 0) i = 0; else if (i >= nr - 1) i = nr - 2;
+        if (j < 0) j = 0; else if (j >= nphi - 1) j = nphi - 2;
+        const double xx0 = r0 + (double)i * dr, yy0 = phi0 + (double)j * dphi;
+        const double tx = (r - xx0) / dr, ty = (loc - yy0) / dphi;
+        const double w00 = (1.0 - tx) * (1.0 - ty), w01 = (1.0 - tx) * ty, w10 = tx * (1.0 - ty), w11 = tx * ty;
+        const int64_t a = i * nphi + j, b = (i + 1) * nphi + j;
+        const double er = w00 * Er_map[a] + w01 * Er_map[a + 1] + w10 * Er_map[b] + w11 * Er_map[b + 1];
+        const double ez = w00 * Ez_map[a] + w01 * Ez_map[a + 1] + w10 * Ez_map[b] + w11 * Ez_map[b + 1];
+        const double ef = w00 * Ephi_map[a] + w01 * Ephi_map[a + 1] + w10 * Ephi_map[b] + w11 * Ephi_map[b + 1];
+        Er[k] = rf_amp * er;
+        Ez[k] = rf_amp * ez * z;
+        Efi[k] = rf_amp * ef;
+        gap_idx[k] = hit;
+    }
+}
+
 /* cylindrical_to_cartesian, src/FFAG_MathTools.py:1969-2014.  cart holds
  * 12 planes of n: x y z vx vy vz Ex Ey Ez Bx By Bz. */
 void ora_cyl2cart(const double *M, int64_t n, const double *E_r, const double *E_z, const double *E_fi,
@@ -606,6 +650,11 @@ typedef struct {
     int32_t aperture_enable;
     const double *aperture;
     double q, c, m0, E0;
+    /* spiral gaps (gap_model 1): gap_theta holds the gap azimuths */
+    int32_t gap_model;
+    int64_t emap_nr, emap_nphi;
+    double emap_r0, emap_dr, emap_phi0, emap_dphi;
+    const double *emap_er, *emap_ez, *emap_ephi;
 } ora_machine;
 
 static void freq_at(const ora_machine *m, double t, double *f, double *v) {
@@ -656,6 +705,15 @@ static void field_eval_c(const ora_machine *m, const double *M, int64_t n, ora_s
     freq_at(m, t, f, v);
     if (!m->emap_enable) {
         memset(s->Er, 0, n * 8); memset(s->Efi, 0, n * 8); memset(s->Ez, 0, n * 8);
+    }
+    if (m->gap_model == 1) {
+        if (m->emap_enable)
+            ora_emap_spiral(M, n, m->gap_theta, m->n_gap, *v * (1.0 / m->gap_width), m->emap_r0, m->emap_dr,
+                            m->emap_nr, m->emap_phi0, m->emap_dphi, m->emap_nphi, m->emap_er, m->emap_ez,
+                            m->emap_ephi, s->Er, s->Ez, s->Efi, s->gap);
+        else
+            for (int64_t k = 0; k < n; ++k) s->gap[k] = 0;
+        return;
     }
     for (int g = 0; g < m->n_gap; ++g)
         ora_emap_gap(M, n, (1.0 / m->gap_width) * *v, m->gap_theta[g] + PI_D / 2.0, m->gap_px + 4 * g,

@@ -103,11 +103,21 @@ class Machine:
     harm_rmin: np.ndarray = field(default_factory=lambda: np.zeros(0))
     harm_rmax: np.ndarray = field(default_factory=lambda: np.zeros(0))
     aperture: np.ndarray | None = None
+    # spiral gaps (FFAG_EField_spiral attributes, src/FFAG_Field.py:379-430):
+    # gap_model "spiral" with maps (nr, nphi) and their axes
+    gap_model: str = "fixed"
+    emap_er: np.ndarray | None = None
+    emap_ez: np.ndarray | None = None
+    emap_ephi: np.ndarray | None = None
+    emap_r_axis: np.ndarray | None = None
+    emap_phi_axis: np.ndarray | None = None
 
     def gap_regions(self):
         """compute_acc_region + per-gap bounding boxes,
         src/FFAG_Field.py:182-192, 236-261."""
         out = []
+        if self.gap_model == "spiral":
+            return out
         for th in np.atleast_1d(self.gap_azimuths):
             cs = np.array([self.e_rmin * np.cos(th), self.e_rmin * np.sin(th)])
             ce = np.array([self.e_rmax * np.cos(th), self.e_rmax * np.sin(th)])
@@ -179,6 +189,17 @@ def field_eval(mc: Machine, M, sc: Scratch, t, use_raw_fi_for_emap=True):
         sc.Er.fill(0.0)
         sc.Efi.fill(0.0)
         sc.Ez.fill(0.0)
+    if mc.gap_model == "spiral":
+        if mc.emap_enable:
+            ra, pa = mc.emap_r_axis, mc.emap_phi_axis
+            gaz = np.ascontiguousarray(np.atleast_1d(mc.gap_azimuths), dtype=np.float64)
+            L.ora_emap_spiral(_p(M), _i64(n), _p(gaz), _i(gaz.size), _d(v0 * (1.0 / mc.gap_width)),
+                              _d(ra[0]), _d(ra[1] - ra[0]), _i64(ra.size), _d(pa[0]), _d(pa[1] - pa[0]),
+                              _i64(pa.size), _p(mc.emap_er), _p(mc.emap_ez), _p(mc.emap_ephi),
+                              _p(sc.Er), _p(sc.Ez), _p(sc.Efi), _p(sc.gap))
+        else:
+            sc.gap.fill(0)
+        return f_rf, v0
     for g, (px, py, box, th) in enumerate(mc.gap_regions()):
         L.ora_emap_gap(_p(M), _i64(n), _d((1.0 / mc.gap_width) * v0), _d(th + np.pi / 2.0), _p(px), _p(py),
                        _i(px.size), _d(box[0]), _d(box[1]), _d(box[2]), _d(box[3]),
@@ -448,7 +469,10 @@ class _OraMachine(ctypes.Structure):
                 ("gap_theta", ctypes.c_void_p), ("gap_px", ctypes.c_void_p), ("gap_py", ctypes.c_void_p),
                 ("gap_box", ctypes.c_void_p), ("rf_shift", ctypes.c_void_p), ("freq_curve", ctypes.c_void_p),
                 ("n_freq", _i64), ("freq_cols", ctypes.c_int32), ("aperture_enable", ctypes.c_int32),
-                ("aperture", ctypes.c_void_p), ("q", _d), ("c", _d), ("m0", _d), ("E0", _d)]
+                ("aperture", ctypes.c_void_p), ("q", _d), ("c", _d), ("m0", _d), ("E0", _d),
+                ("gap_model", ctypes.c_int32), ("emap_nr", _i64), ("emap_nphi", _i64), ("emap_r0", _d),
+                ("emap_dr", _d), ("emap_phi0", _d), ("emap_dphi", _d), ("emap_er", ctypes.c_void_p),
+                ("emap_ez", ctypes.c_void_p), ("emap_ephi", ctypes.c_void_p)]
 
 
 def _c_machine(mc: Machine):
@@ -490,6 +514,15 @@ def _c_machine(mc: Machine):
         m.aperture_enable = 1
         m.aperture = arr(mc.aperture)
     m.q, m.c, m.m0, m.E0 = Q, C, M0, E0_J
+    if mc.gap_model == "spiral":
+        gaz = np.atleast_1d(mc.gap_azimuths)
+        m.gap_model = 1
+        m.n_gap = gaz.size
+        m.gap_theta = arr(gaz)
+        ra, pa = mc.emap_r_axis, mc.emap_phi_axis
+        m.emap_nr, m.emap_nphi = ra.size, pa.size
+        m.emap_r0, m.emap_dr, m.emap_phi0, m.emap_dphi = ra[0], ra[1] - ra[0], pa[0], pa[1] - pa[0]
+        m.emap_er, m.emap_ez, m.emap_ephi = arr(mc.emap_er), arr(mc.emap_ez), arr(mc.emap_ephi)
     return m, keep
 
 

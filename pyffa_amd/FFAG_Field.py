@@ -213,6 +213,72 @@ class FFAG_EField_new:
         return params, data
 
 
+class FFAG_EField_spiral:
+    """Spiral RF gaps (src/FFAG_Field.py:379-550): the same frequency-curve
+    file as FFAG_EField_new with ``gap_model = spiral``, plus three maps
+    (``Er_map_file``, ``Ez_map_file``, ``Ephi_map_file``; first row the
+    continuous azimuth axis, first column the radius) per unit V0/gap_width.
+    The field evaluation runs on the GPU (gap_model 1 of the machine
+    description); ``device_emap`` packs the maps for it."""
+
+    def __init__(self, freq_curve_path, EnableFlag):
+        header, data = self.ParseEmapFile(freq_curve_path)
+        self.GapType = "spiral"
+        self.acc_phi0 = np.deg2rad(float(header["acc_phi_paint"]))
+        self.gap_azimuths = np.deg2rad(np.atleast_1d(header["gap_azimuth"]).astype(np.float64))
+        self.gap_width = float(header["gap_width"])
+        self.freq_curve = data
+        self.harmonic = int(header["harmonic"])
+        self.EnableFlag = bool(EnableFlag)
+        self.rf_shift = np.deg2rad(np.atleast_1d(header["rf_shift"]).astype(np.float64))
+        d = os.path.dirname(os.path.abspath(freq_curve_path))
+        self.Er_map, self.r_axis, self.phi_axis = self.LoadSpiralMap(
+            os.path.join(d, header.get("Er_map_file", "Er_coef.txt")))
+        self.Ezcoef_map, _, _ = self.LoadSpiralMap(os.path.join(d, header.get("Ez_map_file", "Ez_coef.txt")))
+        self.Ephi_map, _, _ = self.LoadSpiralMap(os.path.join(d, header.get("Ephi_map_file", "Ephi_coef.txt")))
+        self.r0 = self.r_axis[0]
+        self.dr = self.r_axis[1] - self.r_axis[0]
+        self.nr = len(self.r_axis)
+        self.phi0 = self.phi_axis[0]
+        self.dphi = self.phi_axis[1] - self.phi_axis[0]
+        self.nphi = len(self.phi_axis)
+        self.map_rmin, self.map_rmax = self.r_axis[0], self.r_axis[-1]
+        self.map_phimin, self.map_phimax = self.phi_axis[0], self.phi_axis[-1]
+        self.acc_Etot_uniform = 1.0 / self.gap_width
+        # gap intersections with the closed orbits, by energy
+        order = np.argsort(np.atleast_1d(header.get("Ek_gap", [0.0])).astype(np.float64))
+        self.Ek_gap = np.atleast_1d(header.get("Ek_gap", [0.0])).astype(np.float64)[order]
+        self.r_gap = np.atleast_1d(header.get("r_gap", [0.0])).astype(np.float64)[order]
+        self.phi_gap = np.deg2rad(np.atleast_1d(header.get("phi_gap_deg", [0.0])).astype(np.float64))[order]
+        self.phi_gap_unwrapped = np.deg2rad(
+            np.atleast_1d(header.get("phi_gap_unwrapped_deg", [0.0])).astype(np.float64))[order]
+        self._dev_cache = {}
+
+    def r_gap_interp(self, ek):
+        return np.interp(np.asarray(ek, dtype=np.float64), self.Ek_gap, self.r_gap)
+
+    def phi_gap_interp_unwrapped(self, ek):
+        return np.interp(np.asarray(ek, dtype=np.float64), self.Ek_gap, self.phi_gap_unwrapped)
+
+    @staticmethod
+    def LoadSpiralMap(filepath):
+        raw = np.loadtxt(filepath)
+        return raw[1:, 1:].astype(np.float64), raw[1:, 0].astype(np.float64), raw[0, 1:].astype(np.float64)
+
+    Interpolation1D_freq_curve = FFAG_EField_new.Interpolation1D_freq_curve
+    ParseEmapFile = staticmethod(FFAG_EField_new.ParseEmapFile)
+
+    def device_emap(self, device):
+        """The three maps interleaved [r][phi][Er, Ez/z, Ephi], float64, on the
+        device (cached)."""
+        import torch
+        key = str(device)
+        if key not in self._dev_cache:
+            m = np.stack([self.Er_map, self.Ezcoef_map, self.Ephi_map], axis=-1)
+            self._dev_cache[key] = torch.from_numpy(np.ascontiguousarray(m)).to(device)
+        return self._dev_cache[key]
+
+
 class FFAG_BField_Error:
     """Harmonic Bz errors (src/FFAG_Field.py:947-1042)."""
 
@@ -265,7 +331,24 @@ def build_machine(gp, taylor_tensor, taylor_f32: bool) -> "_lib.FfaMachine":
             m.harm_rmin[k] = float(bh.rmin_arr[k])
             m.harm_rmax[k] = float(bh.rmax_arr[k])
     em = gp.Emap
-    if em is not None:
+    if em is not None and getattr(em, "GapType", "fixed") == "spiral":
+        ng = len(em.gap_azimuths)
+        if ng > _lib.FFA_MAX_GAP:
+            raise ValueError(f"at most {_lib.FFA_MAX_GAP} RF gaps are supported")
+        m.rf_enable = 1 if em.EnableFlag else 0
+        m.n_gap = ng
+        m.acc_etot = float(em.acc_Etot_uniform)
+        for g in range(ng):
+            m.gap_theta[g] = float(em.gap_azimuths[g])
+        for g in range(min(len(em.rf_shift), _lib.FFA_MAX_GAP)):
+            m.rf_shift[g] = float(em.rf_shift[g])
+        m.gap_model = 1
+        m.emap_nr, m.emap_nphi = int(em.nr), int(em.nphi)
+        m.emap_r0, m.emap_dr = float(em.r0), float(em.dr)
+        m.emap_phi0, m.emap_dphi = float(em.phi0), float(em.dphi)
+        dev = taylor_tensor.device if taylor_tensor is not None else "cuda"
+        m.emap = em.device_emap(dev).data_ptr()
+    elif em is not None:
         ng = len(em.acc_regions)
         if ng > _lib.FFA_MAX_GAP:
             raise ValueError(f"at most {_lib.FFA_MAX_GAP} RF gaps are supported")

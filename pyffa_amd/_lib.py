@@ -51,6 +51,8 @@ class FfaMachine(ctypes.Structure):
         ("gap_theta", _d * FFA_MAX_GAP),
         ("gap_px", (_d * 4) * FFA_MAX_GAP), ("gap_py", (_d * 4) * FFA_MAX_GAP),
         ("gap_box", (_d * 4) * FFA_MAX_GAP), ("rf_shift", _d * FFA_MAX_GAP),
+        ("gap_model", _i32), ("emap_nr", _i32), ("emap_nphi", _i32),
+        ("emap_r0", _d), ("emap_dr", _d), ("emap_phi0", _d), ("emap_dphi", _d), ("emap", _vp),
         ("aperture_enable", _i32), ("aperture", _d * 4),
         ("q", _d), ("c", _d), ("m0", _d), ("E0", _d),
     ]

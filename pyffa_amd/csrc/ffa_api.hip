@@ -104,6 +104,10 @@ int ffa_set_machine(void *ctx, const ffa_machine *m) {
         return fail(c, FFA_ERR_INVALID, "spiral polynomial degree out of range");
     if (m->n_harm < 0 || m->n_harm > FFA_MAX_HARM) return fail(c, FFA_ERR_INVALID, "n_harm out of range");
     if (m->n_gap < 0 || m->n_gap > FFA_MAX_GAP) return fail(c, FFA_ERR_INVALID, "n_gap out of range");
+    if (m->gap_model != 0 && m->gap_model != 1) return fail(c, FFA_ERR_INVALID, "gap_model must be 0 or 1");
+    if (m->gap_model == 1 && m->rf_enable &&
+        (!m->emap || m->emap_nr < 2 || m->emap_nphi < 2 || !(m->emap_dr > 0.0) || !(m->emap_dphi > 0.0)))
+        return fail(c, FFA_ERR_INVALID, "spiral gaps need a map of >= 2 x 2 nodes with positive steps");
     if (m->max_order < 0) return fail(c, FFA_ERR_INVALID, "max_order negative");
     if (m->taylor_f32 < 0 || m->taylor_f32 > 1) return fail(c, FFA_ERR_INVALID, "taylor_f32 must be 0 or 1");
     if (m->max_order > 0) {
@@ -437,6 +441,20 @@ void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {
             f.gap_py[g][j] = (float)m.gap_py[g][j];
         }
         f.gap_thn[g] = (float)(m.gap_theta[g] + FFA_PI / 2.0);
+        f.gap_az[g] = (float)m.gap_theta[g];
     }
     for (int g = 0; g < FFA_MAX_GAP; ++g) f.rf_shift[g] = (float)m.rf_shift[g];
+    f.gap_model = m.gap_model;
+    if (m.gap_model == 1) {
+        const double phi_max = m.emap_phi0 + m.emap_dphi * (double)(m.emap_nphi - 1);
+        f.emap_nr = m.emap_nr;
+        f.emap_nphi = m.emap_nphi;
+        f.emap_r0 = (float)m.emap_r0;
+        f.emap_inv_dr = (float)(1.0 / m.emap_dr);
+        f.emap_phi0 = (float)m.emap_phi0;
+        f.emap_inv_dphi = (float)(1.0 / m.emap_dphi);
+        f.emap_phi_max = (float)phi_max;
+        f.emap_center = (float)(0.5 * (m.emap_phi0 + phi_max));
+        f.emap = reinterpret_cast<const double *>(m.emap);
+    }
 }

@@ -31,6 +31,63 @@ __device__ __forceinline__ float hw_exp2f(float x) { return __builtin_amdgcn_exp
 
 typedef const __attribute__((address_space(1))) float gfloat;
 
+// atan2 for finite positions away from the origin: t = min/max in [0, 1],
+// atan t = t P(t^2) with P of degree 7 fitted here (weighted least squares on
+// Chebyshev nodes, max error 4e-8 in FP64 and 1.1e-7 rad evaluated in FP32),
+// then the octant and quadrant reflections.  Same branch cut as atan2.
+__device__ __forceinline__ float atan2_f32(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mn * rcpf(mx);
+    const float s = t * t;
+    float p = -0.004054246470332146f;
+    p = p * s + 0.021861769258975983f;
+    p = p * s - 0.05591056868433952f;
+    p = p * s + 0.09642065316438675f;
+    p = p * s - 0.1390857696533203f;
+    p = p * s + 0.1994655430316925f;
+    p = p * s - 0.33329859375953674f;
+    p = p * s + 0.9999993443489075f;
+    float r = t * p;
+    r = ay > ax ? 1.5707963267948966f - r : r;
+    r = x < 0.0f ? 3.141592653589793f - r : r;
+    return copysignf(r, y);
+}
+
+// Spiral RF gaps in FP32 (the FP64 path follows the reference's operation
+// order): first gap whose window, moved by the nearest whole number of
+// turns, holds the azimuth; bilinear fields on (r, local azimuth), weights
+// extrapolating outside the map as the reference's do.
+__device__ __forceinline__ void spiral_gap_f32(const ffa_fconst *__restrict__ k, float r, float th, float z, float acc,
+                                               float &Er, float &Efi, float &Ez, int &gap) {
+    int hit = -1;
+    float loc = 0.0f;
+    for (int g = 0; g < k->n_gap; ++g) {
+        const float ph = th - k->gap_az[g];
+        const float kk = rintf((k->emap_center - ph) * 0.15915494309189535f);
+        const float pl = ph + kk * 6.283185307179586f;
+        if (pl >= k->emap_phi0 && pl <= k->emap_phi_max) { hit = g; loc = pl; break; }
+    }
+    if (hit < 0) return;
+    const int NX = k->emap_nr, NY = k->emap_nphi;
+    const float fx = (r - k->emap_r0) * k->emap_inv_dr, fy = (loc - k->emap_phi0) * k->emap_inv_dphi;
+    int i = (int)floorf(fminf(fmaxf(fx, -1.0f), (float)NX));
+    int j = (int)floorf(fminf(fmaxf(fy, -1.0f), (float)NY));
+    i = i < 0 ? 0 : (i >= NX - 1 ? NX - 2 : i);
+    j = j < 0 ? 0 : (j >= NY - 1 ? NY - 2 : j);
+    const float tx = fx - (float)i, ty = fy - (float)j;
+    const float w00 = (1.0f - tx) * (1.0f - ty), w01 = (1.0f - tx) * ty, w10 = tx * (1.0f - ty), w11 = tx * ty;
+    const double *e00 = k->emap + ((int64_t)i * NY + j) * 3, *e01 = e00 + 3;
+    const double *e10 = k->emap + ((int64_t)(i + 1) * NY + j) * 3, *e11 = e10 + 3;
+    const float er = w00 * (float)e00[0] + w01 * (float)e01[0] + w10 * (float)e10[0] + w11 * (float)e11[0];
+    const float ezc = w00 * (float)e00[1] + w01 * (float)e01[1] + w10 * (float)e10[1] + w11 * (float)e11[1];
+    const float eph = w00 * (float)e00[2] + w01 * (float)e01[2] + w10 * (float)e10[2] + w11 * (float)e11[2];
+    Er = acc * er;
+    Ez = acc * ezc * z;
+    Efi = acc * eph;
+    gap = hit;
+}
+
 __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float z,
                                            float xf, float yf, float v0, double rf, float &Bz, float &Br,
                                            float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
@@ -125,7 +182,8 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
     if (k->rf) {
         int gap = 0;
         const float acc = k->acc * v0;
-        for (int g = 0; g < k->n_gap; ++g) {
+        if (k->gap_model == 1) spiral_gap_f32(k, r, th, z, acc, Er, Efi, Ez, gap);
+        else for (int g = 0; g < k->n_gap; ++g) {
             if (xf < k->gap_box[g][0] || xf > k->gap_box[g][1] || yf < k->gap_box[g][2] || yf > k->gap_box[g][3]) {
                 Er = 0.0f; Efi = 0.0f;
                 continue;
@@ -151,7 +209,7 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
             }
         }
         // the RF factor only matters inside a gap (E = 0 elsewhere)
-        if (Er != 0.0f || Efi != 0.0f) {
+        if (Er != 0.0f || Efi != 0.0f || Ez != 0.0f) {
             const double rfr = rf - FFA_TWO_PI * rint(rf * (1.0 / FFA_TWO_PI));
             sin_rf = __sinf((float)rfr + k->rf_shift[gap]);
         }
@@ -216,7 +274,7 @@ __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_par
         float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
         const float xf = (float)x, yf = (float)y, zf = (float)z;
         const float rr = hw_sqrtf(xf * xf + yf * yf);
-        const float th = atan2f(yf, xf);
+        const float th = atan2_f32(yf, xf);
         if constexpr (SUR) {
             float a, u, c;
             mlp::inputs(md, rr, th, zf, a, u, c);

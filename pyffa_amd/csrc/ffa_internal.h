@@ -27,6 +27,12 @@ struct ffa_fconst {
     float gap_px[FFA_MAX_GAP][4], gap_py[FFA_MAX_GAP][4];
     float gap_thn[FFA_MAX_GAP];    // gap azimuth + pi/2
     float rf_shift[FFA_MAX_GAP];
+    // spiral gaps: window [emap_phi0, emap_phi_max] about each gap azimuth,
+    // bilinear maps on (r, local azimuth)
+    int32_t gap_model, emap_nr, emap_nphi, pad2;
+    float emap_r0, emap_inv_dr, emap_phi0, emap_inv_dphi, emap_phi_max, emap_center;
+    float gap_az[FFA_MAX_GAP];
+    const double *emap;
 };
 
 void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f);

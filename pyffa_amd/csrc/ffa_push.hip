@@ -143,6 +143,52 @@ __device__ __forceinline__ void emap_f64(const ffa_machine &m, double x, double 
     }
 }
 
+// FFAG_EField_spiral.Interpolation2D_EMap (src/FFAG_Field.py:464-498):
+// interpolate_spiral_emap_unique_gap_kernel with shift_to_nearest_window and
+// Bilinear_interp_2D_vect_uniform_3fields (src/FFAG_MathTools.py:939-1135).
+// The first gap whose window holds the raw azimuth (moved by the nearest
+// whole number of turns) acts; its index selects rf_shift.  Indices are
+// clamped to the map, so the weights extrapolate outside it, as there.
+__device__ __forceinline__ void emap_spiral_f64(const ffa_machine &m, double r, double fi, double z, double v0,
+                                                double &Er, double &Efi, double &Ez, int &gap) {
+    Er = 0.0; Efi = 0.0; Ez = 0.0;
+    gap = 0;                     // the reference's -1 (no gap): the field is zero either way
+    const double twopi = 2.0 * FFA_PI;
+    const double phi_min = m.emap_phi0, phi_max = m.emap_phi0 + m.emap_dphi * (double)(m.emap_nphi - 1);
+    const double center = 0.5 * (phi_min + phi_max);
+    int hit = -1;
+    double loc = 0.0;
+    for (int g = 0; g < m.n_gap; ++g) {
+        const double ph = fi - m.gap_theta[g];
+        const double k = rint((center - ph) / twopi);
+        const double pl = ph + k * twopi;
+        if (pl >= phi_min && pl <= phi_max) { hit = g; loc = pl; break; }
+    }
+    if (hit < 0) return;
+    const int NX = m.emap_nr, NY = m.emap_nphi;
+    double fx = (r - m.emap_r0) / m.emap_dr, fy = (loc - m.emap_phi0) / m.emap_dphi;
+    fx = fx > -1e9 ? (fx < 1e9 ? fx : 1e9) : -1e9;
+    fy = fy > -1e9 ? (fy < 1e9 ? fy : 1e9) : -1e9;
+    int64_t i = (int64_t)floor(fx), j = (int64_t)floor(fy);
+    if (i < 0) i = 0; else if (i >= NX - 1) i = NX - 2;
+    if (j < 0) j = 0; else if (j >= NY - 1) j = NY - 2;
+    const double xx0 = m.emap_r0 + (double)i * m.emap_dr, yy0 = m.emap_phi0 + (double)j * m.emap_dphi;
+    const double tx = (r - xx0) / m.emap_dr, ty = (loc - yy0) / m.emap_dphi;
+    const double w00 = (1.0 - tx) * (1.0 - ty), w01 = (1.0 - tx) * ty;
+    const double w10 = tx * (1.0 - ty), w11 = tx * ty;
+    const double *E = reinterpret_cast<const double *>(m.emap);
+    const double *e00 = E + (i * NY + j) * 3, *e01 = e00 + 3;
+    const double *e10 = E + ((i + 1) * NY + j) * 3, *e11 = e10 + 3;
+    const double er = w00 * e00[0] + w01 * e01[0] + w10 * e10[0] + w11 * e11[0];
+    const double ezc = w00 * e00[1] + w01 * e01[1] + w10 * e10[1] + w11 * e11[1];
+    const double eph = w00 * e00[2] + w01 * e01[2] + w10 * e10[2] + w11 * e11[2];
+    const double rf_amp = v0 * m.acc_etot;
+    Er = rf_amp * er;
+    Ez = rf_amp * ezc * z;
+    Efi = rf_amp * eph;
+    gap = hit;
+}
+
 // Field evaluation of one particle (fast_mod + order 0 + Taylor + harmonics
 // + gap field), for injected-and-alive particles.
 __device__ __forceinline__ void fields_f64(const ffa_machine &m, double r, double fi, double z, double sf,
@@ -154,7 +200,10 @@ __device__ __forceinline__ void fields_f64(const ffa_machine &m, double r, doubl
     if (m.max_order >= 1) taylor_add_f64(m, r, fim, z, Bz, Bfi, Br);
     if (m.n_harm > 0) harmonics_f64(m, r, fi, Bz);
     Er = 0.0; Efi = 0.0; Ez = 0.0;
-    if (m.rf_enable) emap_f64(m, r * cf, r * sf, fi, v0, Er, Efi, Ez, gap);
+    if (m.rf_enable) {
+        if (m.gap_model == 1) emap_spiral_f64(m, r, fi, z, v0, Er, Efi, Ez, gap);
+        else emap_f64(m, r * cf, r * sf, fi, v0, Er, Efi, Ez, gap);
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_inject(uint8_t *flags, int64_t b, int64_t e) {

@@ -32,6 +32,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int NB = FFA_SC_BLOCKS;
+constexpr int UNR = 4;       // particles per pass of the reduction loops
 constexpr double EPS0 = 8.854187817e-12;
 
 __device__ __forceinline__ double gamma_of(const ffa_particles &p, int64_t i, int rk, double c, double E0J) {
@@ -58,6 +59,37 @@ __device__ __forceinline__ void block_reduce(double (*sh)[BLOCK], double *v, con
     for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
 }
 
+// Wave-aggregated LDS counter update: lanes holding the same key (the
+// common case after the azimuth ordering, when a wave's particles share a
+// slice) are served by one atomic from their first lane, which returns the
+// group's base; each lane's value is base + its rank among the group's
+// lower lanes.  Up to PEEL distinct keys per wave are grouped; the rest fall
+// back to one atomic per lane.  Returns the counter's value before this
+// lane's increment.
+constexpr int PEEL = 4;
+
+__device__ __forceinline__ int32_t wave_counter_add(int32_t *ctr, int key, bool valid) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    bool todo = valid;
+    int32_t out = 0;
+    for (int k = 0; k < PEEL; ++k) {
+        const uint64_t act = __ballot(todo);
+        if (act == 0ull) break;                                  // wave-uniform
+        const int leader = __builtin_ctzll(act);
+        const int k0 = __shfl(key, leader);
+        const bool mine = todo && key == k0;
+        const uint64_t grp = __ballot(mine);
+        int32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[k0], (int32_t)__popcll(grp));
+        base = __shfl(base, leader);
+        if (mine) out = base + (int32_t)__popcll(grp & below);
+        todo = todo && !mine;
+    }
+    if (todo) out = atomicAdd(&ctr[key], 1);
+    return out;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_sc_fill(int32_t *slc, int ns) {
     for (int s = threadIdx.x; s < ns; s += BLOCK) slc[s] = 0;
 }
@@ -68,16 +100,32 @@ __global__ __launch_bounds__(BLOCK) void k_sc_stage1(ffa_particles p, int rk, do
     __shared__ double sh[10][BLOCK];
     const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
     double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
-        if (!(p.flags[i] & FFA_F_ALIVE)) continue;
-        v[0] += gamma_of(p, i, rk, c, E0J);
-        v[1] += 1.0;
-        const double fm = mod2pi(p.fi[i]);
-        for (int k = 0; k < 4; ++k) {
-            const double a = fm + shifts[k], b = 2.0 * FFA_PI;
-            const double q = a - b * floor(a / b);
-            v[2 + k] = q < v[2 + k] ? q : v[2 + k];
-            v[6 + k] = q > v[6 + k] ? q : v[6 + k];
+    // UNR particles per pass with their loads issued together; a thread
+    // still visits its particles i, i + S, i + 2S, ... in that order
+    const int64_t S = (int64_t)NB * BLOCK;
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        bool ok[UNR];
+        double g[UNR], fi[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int64_t i = i0 + u * S;
+            ok[u] = i < p.n && (p.flags[i < p.n ? i : 0] & FFA_F_ALIVE);
+            const int64_t j = ok[u] ? i : 0;
+            g[u] = ok[u] ? gamma_of(p, j, rk, c, E0J) : 0.0;
+            fi[u] = ok[u] ? p.fi[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (!ok[u]) continue;
+            v[0] += g[u];
+            v[1] += 1.0;
+            const double fm = mod2pi(fi[u]);
+            for (int k = 0; k < 4; ++k) {
+                const double a = fm + shifts[k], b = 2.0 * FFA_PI;
+                const double q = a - b * floor(a / b);
+                v[2 + k] = q < v[2 + k] ? q : v[2 + k];
+                v[6 + k] = q > v[6 + k] ? q : v[6 + k];
+            }
         }
     }
     const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
@@ -162,39 +210,64 @@ __device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, 
     return (1 - tx) * (1 - ty) * z00 + (1 - tx) * ty * z01 + tx * (1 - ty) * z10 + tx * ty * z11;
 }
 
-// SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
-// Slice histogram: per-block counts in LDS, one global atomic per non-empty
-// (block, slice) pair (ns <= FFA_SC_MAX_NS).
+// SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531)
+// in two kernels: k_sc_locate per particle, k_sc_moments for the sums and
+// the slice histogram.
 constexpr int MAX_NS_LDS = 4096;
 
-__global__ __launch_bounds__(BLOCK) void k_sc_stage2(ffa_particles p, ffa_sc_params sp, double c, double E0J,
+// Per particle (one per thread, so the SEO table reads of many particles are
+// in flight at once): x = r - r0(Ek, phi), pr0 and the slice index.
+__global__ __launch_bounds__(BLOCK) void k_sc_locate(ffa_particles p, ffa_sc_params sp, double c, double E0J,
                                                     const ffa_sc_result *res, double *px, double *ppr0,
-                                                    int32_t *pslice, int32_t *slc, double *part) {
+                                                    int32_t *pslice) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    if (!(p.flags[i] & FFA_F_ALIVE)) { pslice[i] = -1; return; }
+    const double fs = res->fi_s, step = res->slice_step;
+    const double g = gamma_of(p, i, sp.rk, c, E0J);
+    const double fm = mod2pi(p.fi[i]);
+    const double ek = (g - 1) * sp.E0_MeV;
+    const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+    const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+    double f = fm;
+    if (f < fs) f += FFA_TWO_PI;
+    double fsl = (f - fs) / step;
+    fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
+    int64_t s = (int64_t)fsl;
+    if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
+    px[i] = p.r[i] - r0;
+    ppr0[i] = pr0;
+    pslice[i] = (int32_t)s;
+}
+
+// Moments of the survivors (sum x, z, x^2, z^2) and the slice histogram:
+// per-block counts in LDS, one global atomic per non-empty (block, slice)
+// pair (ns <= FFA_SC_MAX_NS).  Fixed block partition, so the sums are
+// reproducible run to run.
+__global__ __launch_bounds__(BLOCK) void k_sc_moments(ffa_particles p, ffa_sc_params sp, const double *px,
+                                                     const int32_t *pslice, int32_t *slc, double *part) {
     __shared__ double sh[4][BLOCK];
     __shared__ int32_t hist[MAX_NS_LDS];
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
     __syncthreads();
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const double fs = res->fi_s, step = res->slice_step;
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
-        if (!(p.flags[i] & FFA_F_ALIVE)) { pslice[i] = -1; continue; }
-        const double g = gamma_of(p, i, sp.rk, c, E0J);
-        const double fm = mod2pi(p.fi[i]);
-        const double ek = (g - 1) * sp.E0_MeV;
-        const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-        const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-        const double x = p.r[i] - r0, z = p.z[i];
-        v[0] += x; v[1] += z; v[2] += x * x; v[3] += z * z;
-        double f = fm;
-        if (f < fs) f += FFA_TWO_PI;
-        double fsl = (f - fs) / step;
-        fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
-        int64_t s = (int64_t)fsl;
-        if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-        px[i] = x;
-        ppr0[i] = pr0;
-        pslice[i] = (int32_t)s;
-        atomicAdd(&hist[s], 1);
+    const int64_t S = (int64_t)NB * BLOCK;
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        int s[UNR];
+        double x[UNR], z[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int64_t i = i0 + u * S;
+            s[u] = i < p.n ? pslice[i] : -1;
+            const int64_t j = s[u] >= 0 ? i : 0;
+            x[u] = s[u] >= 0 ? px[j] : 0.0;
+            z[u] = s[u] >= 0 ? p.z[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (s[u] >= 0) { v[0] += x[u]; v[1] += z[u]; v[2] += x[u] * x[u]; v[3] += z[u] * z[u]; }
+            wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
+        }
     }
     const int op[4] = {0, 0, 0, 0};
     block_reduce<4>(sh, v, op);       // contains __syncthreads: hist is complete
@@ -288,9 +361,13 @@ __global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params
     if (res->bad_grid != 0.0) return;
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
-        const int s = pslice[i];
-        if (s >= 0) atomicAdd(&hist[s], 1);
+    const int64_t S = (int64_t)NB * BLOCK;
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        int s[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) s[u] = i0 + u * S < p.n ? pslice[i0 + u * S] : -1;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK) {
@@ -298,11 +375,22 @@ __global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params
         hist[k] = c ? atomicAdd(&cursor[k], c) : 0;      // base of this block's range
     }
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < p.n; i += (int64_t)NB * BLOCK) {
-        const int s = pslice[i];
-        if (s < 0) continue;
-        const int pos = atomicAdd(&hist[s], 1);
-        rec[pos] = cic_record(px[i], p.z[i], res, sp.nx, sp.nz, sp.bits);
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        int s[UNR];
+        double x[UNR], z[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int64_t i = i0 + u * S;
+            s[u] = i < p.n ? pslice[i] : -1;
+            const int64_t j = s[u] >= 0 ? i : 0;
+            x[u] = s[u] >= 0 ? px[j] : 0.0;
+            z[u] = s[u] >= 0 ? p.z[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int pos = wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
+            if (s[u] >= 0) rec[pos] = cic_record(x[u], z[u], res, sp.nx, sp.nz, sp.bits);
+        }
     }
 }
 
@@ -785,7 +873,10 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     hipLaunchKernelGGL(k_sc_fill, dim3(1), dim3(BLOCK), 0, st, slc, sp->ns);
     hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, sp->c, E0J, part);
     hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
-    hipLaunchKernelGGL(k_sc_stage2, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0, pslice, slc, part);
+    if (p->n > 0)
+        hipLaunchKernelGGL(k_sc_locate, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0,
+                           pslice);
+    hipLaunchKernelGGL(k_sc_moments, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, px, pslice, slc, part);
     hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart, cursor);
     hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, cursor, rec);
     const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile

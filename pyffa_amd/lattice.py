@@ -413,6 +413,80 @@ def write_emap(path, *, gap_azimuth_deg=(0.0, 180.0), harmonic=1, v0_volt=100e3,
         np.savetxt(fh, np.column_stack([t, f, v, e]), fmt="%.12e", delimiter="\t")
 
 
+def write_spiral_emap(path, *, gap_azimuth_deg=(0.0, 180.0), harmonic=1, v0_volt=100e3,
+                      f_start=None, f_end=None, t_start=-200e-9, t_end=1e-3, n_t=4001,
+                      gap_width=0.05, spiral_deg=35.0, r_ref=8.5, rmin=8.0, rmax=9.3, nr=131, nphi=241,
+                      acc_phi_deg=0.0, ek_mev=300.0):
+    """Write a spiral-gap RF file (``gap_model = spiral``, header keys of
+    examples/main_Emap.py:754-787) and its three maps next to it.
+
+    The gap follows the spiral line phi_c(r) = tan(spiral) ln(r / r_ref) in
+    the local azimuth; across it the accelerating field is a Gaussian of rms
+    width gap_width / 2 whose integral over the normal distance is
+    gap_width (so the crossing gains V0), pointing along the line's normal
+    (Er, Ephi) = (-sin, cos)(spiral).  Ez / z = -div_perp E keeps the field
+    divergence-free to first order in z.  Maps are per unit V0 / gap_width."""
+    ngap = len(gap_azimuth_deg)
+    rf_shift = [-360.0 * harmonic * i / ngap for i in range(ngap)]
+    ta = np.tan(np.deg2rad(spiral_deg))
+    r_axis = np.linspace(rmin, rmax, nr)
+    pc = ta * np.log(r_axis / r_ref)
+    margin = 4.0 * gap_width / rmin
+    phi_axis = np.linspace(pc.min() - margin, pc.max() + margin, nphi)
+    R, PHI = np.meshgrid(r_axis, phi_axis, indexing="ij")
+    ca, sa = np.cos(np.deg2rad(spiral_deg)), np.sin(np.deg2rad(spiral_deg))
+    d = (PHI - ta * np.log(R / r_ref)) * R * ca               # normal distance from the line
+    sig = 0.5 * gap_width
+    prof = np.exp(-0.5 * (d / sig) ** 2) * gap_width / (np.sqrt(2.0 * np.pi) * sig)
+    er, ephi = -sa * prof, ca * prof
+    dr, dphi = r_axis[1] - r_axis[0], phi_axis[1] - phi_axis[0]
+    ezc = -(np.gradient(R * er, dr, axis=0) / R + np.gradient(ephi, dphi, axis=1) / R)
+    d0 = os.path.dirname(path)
+    if d0:
+        os.makedirs(d0, exist_ok=True)
+    names = {"Er_map_file": "Er_coef.txt", "Ez_map_file": "Ez_coef.txt", "Ephi_map_file": "Ephi_coef.txt"}
+    for key, arr in (("Er_map_file", er), ("Ez_map_file", ezc), ("Ephi_map_file", ephi)):
+        tab = np.zeros((nr + 1, nphi + 1))
+        tab[0, 1:] = phi_axis
+        tab[1:, 0] = r_axis
+        tab[1:, 1:] = arr
+        np.savetxt(os.path.join(d0, names[key]), tab, fmt="%.17e")
+    t = np.linspace(t_start, t_end, n_t)
+    f = np.linspace(f_start, f_end, n_t)
+    ek_nodes = [ek_mev - 10.0, ek_mev + 10.0]
+    hdr = [
+        "gap_model = spiral",
+        f"Ngap = {ngap}",
+        f"acc_voltage_paint = {v0_volt}",
+        f"acc_voltage_acc = {v0_volt}",
+        f"acc_phi_paint = {float(acc_phi_deg)}",
+        f"acc_phi_acc = {float(acc_phi_deg)}",
+        f"gap_azimuth = {[float(g) for g in gap_azimuth_deg]}",
+        f"rf_shift = {[float(s) for s in rf_shift]}",
+        f"gap_width = {float(gap_width)}",
+        f"E_rmin = {float(rmin)}",
+        f"E_rmax = {float(rmax)}",
+        f"harmonic = {int(harmonic)}",
+        f"spiral_alpha_deg = {float(spiral_deg)}",
+        f"spiral_r_ref = {float(r_ref)}",
+        "spiral_s_ref_deg = 0.0",
+        "spiral_gap_node_count = 2",
+        f"Ek_gap = {ek_nodes}",
+        f"r_gap = {[float(r_ref), float(r_ref)]}",
+        "phi_gap_deg = [0.0, 0.0]",
+        "phi_gap_unwrapped_deg = [0.0, 0.0]",
+        f"dt_uniform = {(t[1] - t[0]):.6e}",
+        f"Er_map_file = {names['Er_map_file']}",
+        f"Ez_map_file = {names['Ez_map_file']}",
+        f"Ephi_map_file = {names['Ephi_map_file']}",
+        "#Time (s)        Frequency (Hz)    V0 (V)    Energy (MeV)",
+    ]
+    with open(path, "w") as fh:
+        fh.write("\n".join(hdr) + "\n")
+        np.savetxt(fh, np.column_stack([t, f, np.full(n_t, v0_volt), np.full(n_t, ek_mev)]), fmt="%.12e",
+                   delimiter="\t")
+
+
 # ---------------------------------------------------------------------------
 # Particle distribution
 # ---------------------------------------------------------------------------

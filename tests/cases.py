@@ -40,9 +40,15 @@ def seo(kind="stable"):
     return Lt.seo_tables(lattice(kind), np.arange(285.0, 316.0, 5.0), n_theta=1000)
 
 
-def emap_file(lat, tmpdir, v0=100e3, gap_width=0.2, ramp=(1.0, 1.002)):
+def emap_file(lat, tmpdir, v0=100e3, gap_width=0.2, ramp=(1.0, 1.002), model="fixed"):
     co = Lt.closed_orbit(lat, lat.ek_ref_mev, n_theta=1000)
     f0 = 1.0 / co["T0"]
+    if model == "spiral":
+        path = os.path.join(tmpdir, "spiral", "FrequencyCurve_spiral.txt")
+        Lt.write_spiral_emap(path, gap_azimuth_deg=(15.0, 195.0), harmonic=1, v0_volt=v0, f_start=f0 * ramp[0],
+                             f_end=f0 * ramp[1], t_start=-300e-9, t_end=2e-3, n_t=2001, gap_width=0.05,
+                             rmin=lat.rmin, rmax=lat.rmax)
+        return path
     path = os.path.join(tmpdir, "FrequencyCurve_2gaps.txt")
     Lt.write_emap(path, gap_azimuth_deg=(0.0, 180.0), harmonic=1, v0_volt=v0, f_start=f0 * ramp[0],
                   f_end=f0 * ramp[1], t_start=-300e-9, t_end=2e-3, n_t=2001, gap_width=gap_width,
@@ -64,7 +70,7 @@ def bmap_config(lat):
 
 def machines(kind="stable", max_order=2, rf=False, aperture=None, harmonics=None, tmpdir=None):
     """Returns (oracle Machine, product FFAG_GlobalParameters)."""
-    from pyffa_amd.FFAG_Field import FFAG_Bfield_analytical, FFAG_BField_Error, FFAG_EField_new
+    from pyffa_amd.FFAG_Field import FFAG_Bfield_analytical, FFAG_BField_Error, FFAG_EField_new, FFAG_EField_spiral
     from pyffa_amd.FFAG_ParasAndConversion import FFAG_GlobalParameters
     lat = lattice(kind)
     bz, br, bfi = maps(kind, max(max_order, 1))
@@ -80,14 +86,21 @@ def machines(kind="stable", max_order=2, rf=False, aperture=None, harmonics=None
     if rf:
         tmpdir = tmpdir or tempfile.mkdtemp()
         path = emap_file(lat, tmpdir, **(rf if isinstance(rf, dict) else {}))
-        em = FFAG_EField_new(path, True)
+        spiral = isinstance(rf, dict) and rf.get("model") == "spiral"
+        em = FFAG_EField_spiral(path, True) if spiral else FFAG_EField_new(path, True)
         gp.AddEMap(em)
         mc.emap_enable = True
         mc.gap_azimuths = em.gap_azimuths.copy()
         mc.gap_width = em.gap_width
-        mc.e_rmin, mc.e_rmax = em.rmin, em.rmax
         mc.rf_shift = em.rf_shift.copy()
         mc.freq_curve = em.freq_curve.copy()
+        if spiral:
+            mc.gap_model = "spiral"
+            mc.emap_er, mc.emap_ez, mc.emap_ephi = (np.ascontiguousarray(a) for a in
+                                                    (em.Er_map, em.Ezcoef_map, em.Ephi_map))
+            mc.emap_r_axis, mc.emap_phi_axis = em.r_axis.copy(), em.phi_axis.copy()
+        else:
+            mc.e_rmin, mc.e_rmax = em.rmin, em.rmax
     if aperture is not None:
         gp.AddAperture(*aperture)
         mc.aperture = np.array(aperture, dtype=np.float64)

@@ -266,3 +266,57 @@ def test_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
     # FP32 field rounding moves trajectories by microns over 2000 steps (beam rms ~2 cm)
     assert abs(ma["r"] - mb["r"]) < 1e-5 and abs(ma["z"] - mb["z"]) < 1e-5
     assert abs(ma["er"] / mb["er"] - 1) < 1e-3 and abs(ma["ez"] / mb["ez"] - 1) < 1e-3
+
+
+def _spiral_bunch(mc, n, seed, solver="boris", kind="prescribed"):
+    """Half the particles placed across the spiral gap lines (in the field)."""
+    M = cases.bunch(kind, n=n, seed=seed, solver=solver)
+    rng = np.random.default_rng(seed + 100)
+    k = np.arange(0, n, 2)
+    g = rng.integers(0, mc.gap_azimuths.size, k.size)
+    r = M[k, 0]
+    ta = np.tan(np.deg2rad(35.0))                      # the generator's spiral angle and r_ref = 8.5 m
+    M[k, 4] = mc.gap_azimuths[g] + ta * np.log(r / 8.5) + rng.normal(0.0, 0.03 / r)
+    return M
+
+
+@pytest.mark.parametrize("solver", ["boris", "rk4"])
+def test_spiral_gaps_parity(torch_cuda, tmp_path, solver):
+    """Spiral RF gaps (FFAG_EField_spiral) in the FP64 Boris and RK4 pushers
+    against the oracle, 1 and 20 steps."""
+    import dataclasses
+    print(f"[case] spiral gaps, {solver}", flush=True)
+    mc, gp = cases.machines("prescribed", max_order=2, rf=dict(v0=300e3, model="spiral"), tmpdir=str(tmp_path))
+    M = _spiral_bunch(mc, 4096, 11, solver)
+    O.lib().ora_set_threads(8)
+    for nsteps, tol in ((1, 1e-12), (20, 1e-11)):
+        ref = M.copy()
+        O.run_c(mc, ref, 0.0, DT, nsteps, rk4=solver == "rk4")
+        B = _gpu_run(M.copy(), gp, n_steps=nsteps, solver=solver, kind="prescribed")
+        err = cases.rel_err(B.LocalBunch, ref)
+        print(nsteps, err, flush=True)
+        assert max(err.values()) < tol, (nsteps, err)
+    # the gaps acted: against the same run with the RF off, many particles differ
+    off = M.copy()
+    O.run_c(dataclasses.replace(mc, emap_enable=False), off, 0.0, DT, 20, rk4=solver == "rk4")
+    moved = np.abs(ref[:, 5] - off[:, 5]) > 1e-9 * np.abs(off[:, 5])
+    print("particles kicked by the gaps:", int(moved.sum()), flush=True)
+    assert moved.mean() > 0.1
+
+
+def test_spiral_gaps_fp32_tier_close_to_fp64(torch_cuda, tmp_path):
+    print("[case] spiral gaps, fp32 tier vs fp64, 400 steps", flush=True)
+    mc, gp = cases.machines("stable", max_order=2, rf=dict(v0=300e3, model="spiral"), aperture=cases.APERTURE,
+                            tmpdir=str(tmp_path))
+    M = _spiral_bunch(mc, 4000, 12, kind="stable")
+    B64 = _gpu_run(M.copy(), gp, n_steps=400)
+    B32 = _gpu_run(M.copy(), gp, n_steps=400, precision="fp32")
+    a, b = B32.LocalBunch, B64.LocalBunch
+    both = (a[:, 8] != 0) & (b[:, 8] != 0)
+    assert np.mean(a[:, 8] != b[:, 8]) < 0.005
+    a, b = a[both], b[both]
+    for c, tol in ((0, 1e-6), (4, 1e-6), (5, 1e-6), (1, 1e-3)):
+        scale = np.abs(b[:, c]) + np.sqrt(np.mean(b[:, c] ** 2))
+        e = np.abs(a[:, c] - b[:, c]) / scale
+        print(c, "median", np.median(e), "p99", np.quantile(e, 0.99), flush=True)
+        assert np.quantile(e, 0.99) < tol, (c, np.quantile(e, 0.99))
