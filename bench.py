@@ -248,6 +248,233 @@ def scale_leg(tmp, n, steps):
             "accuracy_fp32_vs_fp64": accuracy(S32, S64, (1 + steps) * SC_STEP)}
 
 
+def orbit_moments(S, seo):
+    """Centroid and rms emittances about the closed orbit: x = r - r0(Ek, phi),
+    x' = vr / v_phi - pr0(Ek, phi) (the SEO tables the SC solve uses; pr0 is
+    the orbit slope dr/(r dphi)), and (z, vz / v_phi), over the alive
+    particles."""
+    import numpy as np
+    fl = S["flags"].cpu().numpy()
+    al = (fl & 2) != 0
+    r, z = S["r"].cpu().numpy()[al], S["z"].cpu().numpy()[al]
+    vr, vz = S["vr"].cpu().numpy()[al], S["vz"].cpu().numpy()[al]
+    vf = r * S["fidot"].cpu().numpy()[al]
+    fi = np.mod(S["fi"].cpu().numpy()[al], 2 * np.pi)
+    g = 1.0 / np.sqrt(1.0 - (vr * vr + vz * vz + vf * vf) / 2.99792458e8 ** 2)
+    ek = (g - 1.0) * 938.2723
+
+    def bil(tab):
+        fx = np.clip((ek - seo.Ek_axis[0]) / (seo.Ek_axis[1] - seo.Ek_axis[0]), 0, seo.Ek_axis.size - 1.000001)
+        fy = np.clip((fi - seo.fi_axis[0]) / (seo.fi_axis[1] - seo.fi_axis[0]), 0, seo.fi_axis.size - 1.000001)
+        i, j = np.floor(fx).astype(int), np.floor(fy).astype(int)
+        tx, ty = fx - i, fy - j
+        return ((1 - tx) * (1 - ty) * tab[i, j] + (1 - tx) * ty * tab[i, j + 1] + tx * (1 - ty) * tab[i + 1, j]
+                + tx * ty * tab[i + 1, j + 1])
+    x = r - bil(seo.r_matrix)
+    xp = vr / vf - bil(seo.pr_matrix)
+    zp = vz / vf
+
+    def emit(a, b):
+        a, b = a - a.mean(), b - b.mean()
+        return float(np.sqrt(np.mean(a * a) * np.mean(b * b) - np.mean(a * b) ** 2))
+    return dict(x_mean=float(x.mean()), z_mean=float(z.mean()), emit_x=emit(x, xp), emit_z=emit(z, zp),
+                alive=int(al.sum()))
+
+
+def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
+    """Tracks the same bunch for ``turns`` turns (SC, RF, aperture) in the
+    requested runs: "fp32" (FP32 tier), "fp64" (FP64 path) and "fp64p" (FP64
+    path from r and z moved by one ulp: the problem's own sensitivity).  The
+    closed-orbit moments are recorded every ``chunk_turns`` turns; the
+    relative emittance differences of fp32 and fp64p against fp64 are
+    reported side by side."""
+    import numpy as np
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    from tests import cases
+    _, gp = machines(2, tmp)
+    seo = cases.seo("stable")
+    M0 = cases.bunch("stable", n=n, seed=3)
+    steps_per_turn = seo.T0[3] / DT
+    total = int(round(turns * steps_per_turn / SC_STEP))
+    n_chunks = max(1, int(round(turns / chunk_turns)))
+    out = {"particles": n, "turns": turns, "sc_grid": list(grid[:3]), "steps": total * SC_STEP,
+           "moments": "about the closed orbit r0(Ek, phi), pr0(Ek, phi) of the SEO tables"}
+    for run in tiers:
+        prec = "fp32" if run == "fp32" else "fp64"
+        M = M0.copy()
+        if run == "fp64p":
+            M[:, 0] = np.nextafter(M[:, 0], np.inf)
+            M[:, 2] = np.nextafter(M[:, 2], np.inf)
+        B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=grid, BunchType="Boris", precision=prec)
+        cases.set_bunch_seo(B)
+        rk = FFAG_RungeKutta()
+        t, done, hist = 0.0, 0, []
+        t0 = time.perf_counter()
+        for c in range(n_chunks):
+            last = total * (c + 1) // n_chunks
+            lp = dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0), time_step=DT,
+                      step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP, precision=prec, verbose=False,
+                      start_step=done * SC_STEP)
+            rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
+            t, done = rk.last_run["t"], last
+            S = {k: B.ordered(k) for k in ("r", "vr", "z", "vz", "fi", "fidot", "flags")}
+            m = orbit_moments(S, seo)
+            hist.append(dict(turns=done * SC_STEP / steps_per_turn, **m))
+            log(f"accuracy {run}: {hist[-1]['turns']:.0f} turns, alive {m['alive']}, emit x {m['emit_x']:.6e} "
+                f"z {m['emit_z']:.6e}, {time.perf_counter() - t0:.0f} s")
+        out[run] = {"history": hist, "seconds": time.perf_counter() - t0}
+        del B, rk
+    if "fp64" in out:
+        ref = out["fp64"]["history"]
+        for run in ("fp32", "fp64p"):
+            if run in out:
+                d = [dict(turns=a["turns"], emit_x_rel=a["emit_x"] / b["emit_x"] - 1.0,
+                          emit_z_rel=a["emit_z"] / b["emit_z"] - 1.0, alive=a["alive"] - b["alive"])
+                     for a, b in zip(out[run]["history"], ref)]
+                out[f"{run}_vs_fp64"] = {"per_chunk": d,
+                                         "max_abs_emit_x_rel": max(abs(e["emit_x_rel"]) for e in d),
+                                         "max_abs_emit_z_rel": max(abs(e["emit_z_rel"]) for e in d),
+                                         "final": d[-1]}
+    return out
+
+
+def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
+    """Taylor-map generation on the full map grid of the 12-sector lattice
+    (r 8.0-9.3 m at 1 mm: 1301 radii; 30 deg at 0.005 deg: 6000 azimuths)
+    with a 35 deg spiral: the FP64 stacks of every order up to max_order
+    (3 (max_order+1) x 1301 x 6000 doubles) and the FP32 tracking table,
+    timed with HIP events on the generator's stream.  The fill is
+    write-bound: its algorithmic bytes are the map bytes it stores.  The
+    CPU figure is the oracle's restatement of the reference generator on
+    ``cpu_rows`` radii at order ``cpu_order``, scaled to the grid."""
+    import numpy as np
+    import torch
+    from numpy.polynomial import Polynomial
+    from pyffa_amd import FFAG_HighOrder as H
+    from tests import cases
+    cfg = cases.lattice("stable").to_config()["Bmap"]
+    r, fi = H.grid_from_config(cfg)
+    s_fit = Polynomial.fit(r, H.calculate_s(r, 0.0, 35.0), 5)
+    conv = H.BmapParams_to_ConvexParams(cfg)
+    coef = tuple(cfg["coefficients_tupel"])
+    out = {"grid": [int(r.size), int(fi.size)], "spiral_deg": 35.0}
+    for mo in orders:
+        t0 = time.perf_counter()
+        gen = H.TaylorMapGenerator(mo, r, fi, conv, s_fit, coef)
+        host_s = time.perf_counter() - t0
+        res = {"host_weights_s": host_s}
+        for layout, dt in (("stacks", torch.float64), ("tracking", torch.float32)):
+            buf = torch.empty(gen.output_shape(layout), dtype=dt, device="cuda")
+            gen.run(layout, dt, out=buf)
+            stream = torch.cuda.current_stream()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(reps):
+                gen.run(layout, dt, out=buf)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            nbytes = buf.numel() * buf.element_size()
+            res[layout] = {"dtype": str(dt).replace("torch.", ""), "bytes": nbytes, "ms": ms,
+                           "write_GBps": nbytes / ms / 1e6, "frac_of_8TBps": nbytes / ms / 1e6 / 8000.0}
+            del buf
+        out[f"order_{mo}"] = res
+        log(f"map generator order {mo}: stacks {res['stacks']['ms']:.3f} ms "
+            f"({res['stacks']['write_GBps']:.0f} GB/s), tracking f32 {res['tracking']['ms']:.3f} ms")
+    if not cpu:
+        return out
+    from oracle import taylor_gen as T
+    poles = [(x0, x1, w0, sg * m) for x0, x1, w0, (m, sg) in conv]
+    t0 = time.perf_counter()
+    T.generate_maps(cpu_order, r[:cpu_rows], fi, poles, s_fit, coef)
+    per_row = (time.perf_counter() - t0) / cpu_rows
+    out["cpu_baseline"] = {"kind": "port", "cores": 1, "order": cpu_order, "sample_rows": cpu_rows,
+                           "seconds_per_row": per_row, "seconds_full_grid": per_row * r.size}
+    log(f"map generator oracle: {per_row:.2f} s per radius at order {cpu_order}")
+    return out
+
+
+def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p")):
+    """Incoherent tunes with space charge: the bunch of ``accuracy_leg`` is
+    tracked ``turns`` + 1.5 turns in each run ("bare": FP64 without SC;
+    "fp64", "fp32": with SC; "fp64p": FP64 with SC from r and z moved by one
+    ulp, the problem's own sensitivity); a PositionDump at azimuth 0 records
+    the first ``n_track`` particles every turn and the FFT tune estimator
+    gives their (qx, qz) about the closed orbit.  The tune shift of a
+    particle is its tune with SC minus its bare tune; the shifts of "fp32"
+    and "fp64p" are compared with those of "fp64" particle by particle."""
+    import numpy as np
+    from pyffa_amd import diagnostics as D
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    from pyffa_amd.FFAG_dump import Dumps, PositionDump
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    from tests import cases
+    _, gp = machines(2, tmp)
+    seo = cases.seo("stable")
+    M0 = cases.bunch("stable", n=n, seed=3)
+    steps_per_turn = seo.T0[3] / DT
+    total = int(round((turns + 1.5) * steps_per_turn / SC_STEP)) * SC_STEP
+    out = {"particles": n, "turns": turns, "tracked": n_track, "sc_grid": list(GRID[:3]), "steps": total,
+           "estimator": "FFT of x + i x'/scale, 8x zero padding, parabolic peak; x about r0(Ek, phi)"}
+    q = {}
+    for run in runs:
+        prec = "fp32" if run == "fp32" else "fp64"
+        M = M0.copy()
+        if run == "fp64p":
+            M[:, 0] = np.nextafter(M[:, 0], np.inf)
+            M[:, 2] = np.nextafter(M[:, 2], np.inf)
+        B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=GRID, BunchType="Boris", precision=prec)
+        cases.set_bunch_seo(B)
+        folder = os.path.join(tmp, f"tune_{run}")
+        pd = PositionDump(0.0, -1.0, 1.0, np.arange(n_track), save_folder=folder, max_records=turns + 8)
+        dumps = Dumps()
+        dumps.add_dump(pd)
+        rk = FFAG_RungeKutta()
+        t0 = time.perf_counter()
+        lp = dict(stop_condition=dict(max_stepsN=total, max_turn=1e9, max_time=1.0), time_step=DT,
+                  step_dumps=dumps, enable_SC=run != "bare", SC_type=1, SC_step=SC_STEP, precision=prec,
+                  verbose=False)
+        rk.rk4_solve_dt_bunch3_withSC_boris(None, 0.0, B, gp, lp)
+        pd.save_all_particles()
+        ids, qx, qz = D.particle_tunes(D.load_position_dump(folder, 0.0), seo, turns=turns)
+        q[run] = dict(zip(ids.tolist(), zip(qx.tolist(), qz.tolist())))
+        out[run] = {"seconds": time.perf_counter() - t0, "particles_analysed": int(ids.size),
+                    "qx_mean": float(qx.mean()), "qz_mean": float(qz.mean()),
+                    "qx_rms": float(qx.std()), "qz_rms": float(qz.std())}
+        log(f"tune {run}: {ids.size} particles, qx {qx.mean():.5f} qz {qz.mean():.5f}, "
+            f"{out[run]['seconds']:.0f} s")
+        del B, rk
+    if "bare" in q:
+        shifts = {}
+        for run in runs:
+            if run == "bare":
+                continue
+            common = sorted(set(q[run]) & set(q["bare"]))
+            d = np.array([[q[run][g][0] - q["bare"][g][0], q[run][g][1] - q["bare"][g][1]] for g in common])
+            shifts[run] = dict(zip(common, d.tolist()))
+            out[run].update({"dqx_mean": float(d[:, 0].mean()), "dqz_mean": float(d[:, 1].mean()),
+                             "dqx_min": float(d[:, 0].min()), "dqz_min": float(d[:, 1].min())})
+        for run in ("fp32", "fp64p"):
+            if run not in shifts or "fp64" not in shifts:
+                continue
+            common = sorted(set(shifts[run]) & set(shifts["fp64"]))
+            a = np.array([shifts[run][g] for g in common])
+            b = np.array([shifts["fp64"][g] for g in common])
+            e = np.abs(a - b)
+            out[f"{run}_vs_fp64"] = {
+                "particles": len(common),
+                "dq_abs_median": [float(np.median(e[:, 0])), float(np.median(e[:, 1]))],
+                "dq_abs_p90": [float(np.quantile(e[:, 0], 0.9)), float(np.quantile(e[:, 1], 0.9))],
+                "dq_abs_max": [float(e[:, 0].max()), float(e[:, 1].max())],
+                "mean_shift_rel": [float(abs(a[:, 0].mean() / b[:, 0].mean() - 1)),
+                                   float(abs(a[:, 1].mean() / b[:, 1].mean() - 1))],
+                "median_shift_rel": [float(abs(np.median(a[:, 0]) / np.median(b[:, 0]) - 1)),
+                                     float(abs(np.median(a[:, 1]) / np.median(b[:, 1]) - 1))]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +491,18 @@ def main():
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--scale-particles", type=int, default=8_000_000)
     ap.add_argument("--scale-steps", type=int, default=10)
+    ap.add_argument("--accuracy-turns", type=float, default=0.0,
+                    help="also run both tiers this many turns and report the moments' deviation (off by default)")
+    ap.add_argument("--accuracy-particles", type=int, default=100_000)
+    ap.add_argument("--accuracy-tiers", default="fp32,fp64,fp64p")
+    ap.add_argument("--only-accuracy", action="store_true", help="run the accuracy leg alone")
+    ap.add_argument("--no-mapgen", action="store_true", help="skip the Taylor-map generator leg")
+    ap.add_argument("--only-mapgen", action="store_true", help="run the Taylor-map generator leg alone")
+    ap.add_argument("--tune-turns", type=int, default=0,
+                    help="also measure incoherent tune shifts over this many turns (off by default)")
+    ap.add_argument("--tune-particles", type=int, default=100_000)
+    ap.add_argument("--tune-tracked", type=int, default=256)
+    ap.add_argument("--only-tune", action="store_true", help="run the tune leg alone")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -279,6 +518,17 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 256 MiB read + 256 MiB written")
     tmp = tempfile.mkdtemp()
+    if args.only_mapgen:
+        print(json.dumps({"map_generator": mapgen_leg()}), flush=True)
+        return
+    if args.only_tune:
+        print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked)}),
+              flush=True)
+        return
+    if args.only_accuracy:
+        acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","))
+        print(json.dumps({"accuracy_long": acc}), flush=True)
+        return
     from tests import cases
     mc, gp = machines(2, tmp)
     M = cases.bunch("stable", n=args.particles, seed=0)
@@ -329,6 +579,13 @@ def main():
     if not args.no_scale:
         out["scale_8m"] = scale_leg(tmp, args.scale_particles, args.scale_steps)
         log(f"8M leg {out['scale_8m']}")
+    if not args.no_mapgen:
+        out["map_generator"] = mapgen_leg(cpu=not args.no_cpu)
+    if args.tune_turns > 0:
+        out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked)
+    if args.accuracy_turns > 0:
+        out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
+                                            args.accuracy_tiers.split(","))
     if not args.no_cpu:
         cb = cpu_baseline(M, mc, min(args.cpu_particles, M.shape[0]), args.cpu_steps)
         log(f"cpu baseline {cb}")

@@ -354,6 +354,47 @@ int ffa_mlp_eval(void *ctx, const ffa_mlp_desc *d, const void *packed, const flo
  * (packed == NULL).  The packed buffer must stay alive while enabled. */
 int ffa_mlp_set(void *ctx, const ffa_mlp_desc *d, const void *packed);
 
+/* ---- Taylor-coefficient map generator ----------------------------------
+ * Replaces the per-radius evaluation loop of Generate_Bmaps_MultiProcess /
+ * compute_taylor_coefficients_group (src/FFAG_HighOrder.py:913-998, driven
+ * by examples/main_Bmap.py:48-75).  For B(r, theta) = A(r) G(theta - s(r)),
+ * every map value is a short sum over flutter derivatives
+ *   map[c][n](r_i, theta_j) = sum_q weights[i][c][n][q] G^(q)(theta_{j - shift_i}),
+ * c = Bz, Br, Bfi; the host expands the polar Laplacian, the Leibniz and
+ * chain rules into the per-radius weights.  The kernel first tabulates
+ * G^(q) on the theta grid: for every pole, height * sum_i C(q, i)
+ * P1_i(sigma1) P2_(q-i)(sigma2) with sigma1 = 1/(1 + exp(-k (theta - x0))),
+ * sigma2 = 1/(1 + exp(k (theta - x1))) and the derivative polynomials P in
+ * sigma given by the caller (sigmoid_nth_derivative, :212-243; q = 0 is
+ * (height sigma1) sigma2, shape_function :246-257), then fills the maps.
+ * The theta index wraps periodically over the n_phi grid points (np.roll of
+ * shift_function, :393-397). */
+typedef struct ffa_taylor_gen {
+    int32_t n_orders;          /* max_order + 1 (1..FFA_TAYLOR_MAX_ORDERS) */
+    int32_t n_q;               /* flutter derivative orders 0..n_q-1; 2 * n_orders */
+    int32_t n_poles;           /* 1..FFA_TAYLOR_MAX_POLES */
+    int32_t layout;            /* 0: stacks [3][n_orders][nr][n_phi], planes Bz, Br, Bfi;
+                                  1: tracking layout [n_phi][nr][3 (n_orders - 1)], orders 1..,
+                                     (Bz, Bfi, Br) per order (the table ffa_set_machine reads) */
+    int32_t out_f32;           /* 0: double output, 1: float (rounded from the double sum) */
+    int32_t pad;
+    int64_t nr, n_phi;
+    const double *theta;       /* device [n_phi] azimuth grid (rad) */
+    const double *poles;       /* device [n_poles][4]: x0, x1, k = 16 / W0, height (sign |polarity|) */
+    const double *sig_poly;    /* device [n_poles][2][n_q][n_q + 1]: coefficients of P_i (highest power
+                                  first, left-padded with zeros) for the rising (k) and falling (-k) edge;
+                                  row i = 0 is sigma itself (0 ... 0 1 0) */
+    const double *weights;     /* device [nr][3][n_orders][n_q] */
+    const int32_t *shift;      /* device [nr] grid steps, taken modulo n_phi */
+    double *flutter;           /* device workspace [n_q][n_phi] (overwritten) */
+    void *out;                 /* device output, layout and type above */
+} ffa_taylor_gen;
+
+#define FFA_TAYLOR_MAX_ORDERS 8
+#define FFA_TAYLOR_MAX_POLES 16
+
+int ffa_taylor_maps(void *ctx, const ffa_taylor_gen *g);
+
 #ifdef __cplusplus
 }
 #endif

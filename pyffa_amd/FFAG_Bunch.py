@@ -70,8 +70,10 @@ class FFAG_Bunch:
         self.BunchType = BunchType
         self.precision = precision
         P = np.ascontiguousarray(ParticlesDistribution, dtype=np.float64)
-        if P.ndim != 2 or P.shape[1] != 16:
+        if P.ndim != 2 or P.shape[1] < 16:
             raise ValueError("ParticlesDistribution must be an (N, 16) array")
+        # a restart file row carries the dump time as a 17th column
+        P = np.ascontiguousarray(P[:, :16])
         self.TotalParticleNum = P.shape[0]
         self.TotalParticleNum_Local = P.shape[0]
         self.marcosize = marcosize

@@ -109,6 +109,16 @@ class FfaMlpLayout(ctypes.Structure):
                 ("loss", ctypes.c_size_t)]
 
 
+class FfaTaylorGen(ctypes.Structure):
+    _fields_ = [("n_orders", _i32), ("n_q", _i32), ("n_poles", _i32), ("layout", _i32), ("out_f32", _i32),
+                ("pad", _i32), ("nr", _i64), ("n_phi", _i64), ("theta", _vp), ("poles", _vp), ("sig_poly", _vp),
+                ("weights", _vp), ("shift", _vp), ("flutter", _vp), ("out", _vp)]
+
+
+FFA_TAYLOR_MAX_ORDERS = 8
+FFA_TAYLOR_MAX_POLES = 16
+
+
 class FfaError(RuntimeError):
     pass
 
@@ -121,7 +131,8 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_push_boris_fused_f32",
             "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
-            "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing")
+            "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing",
+            "ffa_taylor_maps")
 
 
 def load():
@@ -162,6 +173,7 @@ def load():
     lib.ffa_sort_workspace_size.argtypes = [_i64, ctypes.POINTER(ctypes.c_size_t)]
     lib.ffa_sort_particles.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaParticles), _i64, _i32,
                                        _vp, _vp, _vp]
+    lib.ffa_taylor_maps.argtypes = [_vp, ctypes.POINTER(FfaTaylorGen)]
     for name in EXPORTED:
         if name != "ffa_last_error":
             getattr(lib, name).restype = ctypes.c_int

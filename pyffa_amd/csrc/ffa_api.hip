@@ -257,6 +257,25 @@ int ffa_detect_crossing(void *ctx, const double *fi_pre, const uint8_t *flags_pr
                      "ffa_detect_crossing");
 }
 
+int ffa_taylor_maps(void *ctx, const ffa_taylor_gen *g) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !g) return fail(c, FFA_ERR_INVALID, "ctx or generator description is NULL");
+    if (g->n_orders < 1 || g->n_orders > FFA_TAYLOR_MAX_ORDERS)
+        return fail(c, FFA_ERR_INVALID, "n_orders must be in [1, %d]", FFA_TAYLOR_MAX_ORDERS);
+    if (g->n_q != 2 * g->n_orders) return fail(c, FFA_ERR_INVALID, "n_q must be 2 * n_orders");
+    if (g->n_poles < 1 || g->n_poles > FFA_TAYLOR_MAX_POLES)
+        return fail(c, FFA_ERR_INVALID, "n_poles must be in [1, %d]", FFA_TAYLOR_MAX_POLES);
+    if (g->layout != 0 && g->layout != 1) return fail(c, FFA_ERR_INVALID, "layout must be 0 or 1");
+    if (g->layout == 1 && g->n_orders < 2) return fail(c, FFA_ERR_INVALID, "the tracking layout needs n_orders >= 2");
+    if (g->out_f32 != 0 && g->out_f32 != 1) return fail(c, FFA_ERR_INVALID, "out_f32 must be 0 or 1");
+    if (g->nr < 2 || g->n_phi < 2) return fail(c, FFA_ERR_INVALID, "the grid needs at least 2 x 2 points");
+    if (g->nr > 65535) return fail(c, FFA_ERR_INVALID, "nr must be <= 65535");
+    if (g->n_phi > ((int64_t)1 << 23)) return fail(c, FFA_ERR_INVALID, "n_phi must be <= 2^23");
+    if (!g->theta || !g->poles || !g->sig_poly || !g->weights || !g->shift || !g->flutter || !g->out)
+        return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    return hip_check(c, launch_taylor_maps(c, g), "ffa_taylor_maps");
+}
+
 int ffa_sort_workspace_size(int64_t count, size_t *bytes) {
     if (!bytes) return fail(nullptr, FFA_ERR_INVALID, "bytes is NULL");
     if (count < 0 || count > 0xFFFFFFFFLL) return fail(nullptr, FFA_ERR_INVALID, "count must be in [0, 2^32)");

@@ -360,6 +360,20 @@ def write_seo(folder, seo: SEOTables):
         json.dump({"BHarmonics": {"enable": False, "harmonics": []}}, f, indent=2)
 
 
+def write_twiss(folder, seo: SEOTables, beta_r, beta_z, alpha_r=0.0, alpha_z=0.0):
+    """BetaFuncR/Z.txt and AlphaFuncR/Z.txt (first column Ek, first row the
+    azimuth axis) holding the given values at every (Ek, phi): the Twiss
+    inputs the bunch generator reads at the injection point."""
+    os.makedirs(folder, exist_ok=True)
+    for name, v in (("BetaFuncR.txt", beta_r), ("BetaFuncZ.txt", beta_z), ("AlphaFuncR.txt", alpha_r),
+                    ("AlphaFuncZ.txt", alpha_z)):
+        tab = np.zeros((seo.Ek_axis.size + 1, seo.fi_axis.size + 1))
+        tab[0, 1:] = seo.fi_axis
+        tab[1:, 0] = seo.Ek_axis
+        tab[1:, 1:] = v
+        np.savetxt(os.path.join(folder, name), tab, header="first row: fi axis; first column: Ek")
+
+
 def write_bmap(folder, lat: ScalingFFAG, max_order: int):
     """config_Bmap.json + per-order npz stacks, as FFAG_Bfield_analytical loads."""
     os.makedirs(folder, exist_ok=True)
