@@ -519,7 +519,7 @@ def main():
         log("copy calibration: 256 MiB read + 256 MiB written")
     tmp = tempfile.mkdtemp()
     if args.only_mapgen:
-        print(json.dumps({"map_generator": mapgen_leg()}), flush=True)
+        print(json.dumps({"map_generator": mapgen_leg(cpu=not args.no_cpu)}), flush=True)
         return
     if args.only_tune:
         print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked)}),
