@@ -354,6 +354,32 @@ int ffa_mlp_eval(void *ctx, const ffa_mlp_desc *d, const void *packed, const flo
  * (packed == NULL).  The packed buffer must stay alive while enabled. */
 int ffa_mlp_set(void *ctx, const ffa_mlp_desc *d, const void *packed);
 
+/* ---- Closed-orbit tracking in azimuth ----------------------------------
+ * Replaces rk4_solve / rk4_solve_vect over FunctionForSEO(Vect)
+ * (src/FFAG_track.py:582-640, 1597-1695), the tracking under the closed-orbit
+ * search, the tunes and the Twiss functions (FFAG_SearchSEO,
+ * src/FFAG_track.py:1142-1586; FFAG_BetaFuncCalc, src/FFAG_BetaFunc.py:45).
+ * Each particle's state (r, pr, z, pz), pr and pz the direction cosines of
+ * the momentum P, obeys, with theta the independent variable and
+ * pf = sqrt(1 - pr^2 - pz^2),
+ *   dr/dth = r pr / pf,   dpr/dth = pf - (q/P) (r Bz - r pz / pf Bfi),
+ *   dz/dth = r pz / pf,   dpz/dth = (q/P) (r Br - r pr / pf Bfi),
+ * in the order-0 field of the machine set by ffa_set_machine (GetBfields at
+ * theta mod the sector) plus its harmonic errors (at theta mod 2 pi); no
+ * Taylor maps, no RF.  Classic RK4 steps of size h; step s starts at
+ * t0 + s * t_step (the reference's linspace grid). */
+typedef struct ffa_orbit {
+    double *state;             /* device [4][n]: r, pr, z, pz (start in, end out) */
+    const double *q_over_p;    /* device [n]: q / P of each particle */
+    int64_t n;
+    double t0, t_step, h;
+    int32_t nsteps;            /* >= 0 */
+    int32_t pad;
+    double *traj;              /* device [nsteps + 1][4][n] states at every grid point, or NULL */
+} ffa_orbit;
+
+int ffa_orbit_track(void *ctx, const ffa_orbit *o);
+
 /* ---- Taylor-coefficient map generator ----------------------------------
  * Replaces the per-radius evaluation loop of Generate_Bmaps_MultiProcess /
  * compute_taylor_coefficients_group (src/FFAG_HighOrder.py:913-998, driven

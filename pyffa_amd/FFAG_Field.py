@@ -293,6 +293,22 @@ class FFAG_BField_Error:
         self.rmax_arr = np.array([float(h["rmax"]) for h in harmonics_cfg], dtype=np.float64)
 
 
+def mean_order0_bz(bm, r, theta):
+    """Azimuthal mean over ``theta`` of the order-0 Bz = B0 r^k F(theta - s(r))
+    at each radius in ``r`` (host numpy; the mean-field curve the closed-orbit
+    search starts from)."""
+    r = np.atleast_1d(np.asarray(r, dtype=np.float64))
+    th = np.asarray(theta, dtype=np.float64)
+    period = 2 * np.pi / bm.Nsectors
+    s = np.polyval(bm.s_rad_polynomial, r)
+    fs = np.mod(th[None, :] - s[:, None], period)
+    F = np.zeros_like(fs)
+    for x0, x1, w0, mag, sgn in np.asarray(bm.convex_params, dtype=np.float64).reshape(-1, 5):
+        ks = 16.0 / w0
+        F += sgn * abs(mag) / (1.0 + np.exp(-ks * (fs - x0))) / (1.0 + np.exp(ks * (fs - x1)))
+    return bm.B0 * r ** bm.k_value * F.mean(axis=1)
+
+
 def build_machine(gp, taylor_tensor, taylor_f32: bool) -> "_lib.FfaMachine":
     """Pack FFAG_GlobalParameters into the C-ABI machine description."""
     bm = gp.Bmap

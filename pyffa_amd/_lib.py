@@ -115,6 +115,11 @@ class FfaTaylorGen(ctypes.Structure):
                 ("weights", _vp), ("shift", _vp), ("flutter", _vp), ("out", _vp)]
 
 
+class FfaOrbit(ctypes.Structure):
+    _fields_ = [("state", _vp), ("q_over_p", _vp), ("n", _i64), ("t0", _d), ("t_step", _d), ("h", _d),
+                ("nsteps", _i32), ("pad", _i32), ("traj", _vp)]
+
+
 FFA_TAYLOR_MAX_ORDERS = 8
 FFA_TAYLOR_MAX_POLES = 16
 
@@ -132,7 +137,7 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
             "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing",
-            "ffa_taylor_maps")
+            "ffa_taylor_maps", "ffa_orbit_track")
 
 
 def load():
@@ -174,6 +179,7 @@ def load():
     lib.ffa_sort_particles.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaParticles), _i64, _i32,
                                        _vp, _vp, _vp]
     lib.ffa_taylor_maps.argtypes = [_vp, ctypes.POINTER(FfaTaylorGen)]
+    lib.ffa_orbit_track.argtypes = [_vp, ctypes.POINTER(FfaOrbit)]
     for name in EXPORTED:
         if name != "ffa_last_error":
             getattr(lib, name).restype = ctypes.c_int

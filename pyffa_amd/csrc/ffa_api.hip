@@ -257,6 +257,17 @@ int ffa_detect_crossing(void *ctx, const double *fi_pre, const uint8_t *flags_pr
                      "ffa_detect_crossing");
 }
 
+int ffa_orbit_track(void *ctx, const ffa_orbit *o) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !o) return fail(c, FFA_ERR_INVALID, "ctx or orbit description is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (o->n < 0 || o->nsteps < 0) return fail(c, FFA_ERR_INVALID, "negative particle or step count");
+    if (o->n > 0 && (!o->state || !o->q_over_p)) return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (!(o->h == o->h) || !(o->t_step == o->t_step) || !(o->t0 == o->t0))
+        return fail(c, FFA_ERR_INVALID, "t0, t_step and h must be numbers");
+    return hip_check(c, launch_orbit_rk4(c, o), "ffa_orbit_track");
+}
+
 int ffa_taylor_maps(void *ctx, const ffa_taylor_gen *g) {
     ffa_ctx_s *c = as_ctx(ctx);
     if (!c || !g) return fail(c, FFA_ERR_INVALID, "ctx or generator description is NULL");

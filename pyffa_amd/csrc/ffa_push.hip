@@ -422,9 +422,69 @@ __global__ __launch_bounds__(BLOCK) void k_bfield_eval(const ffa_machine *__rest
     out[2 * n + i] = Br;
 }
 
+// FunctionForSEO (src/FFAG_track.py:1597-1638): azimuth-stepped equations of
+// a particle of fixed momentum in the order-0 field plus harmonics.  The
+// azimuth reductions are the reference's Python / numpy float modulo (fmod,
+// exact) rather than the tracker's floor form.
+__device__ __forceinline__ void seo_rhs(const ffa_machine &m, double qp, double t, const double *x, double *d) {
+    const double two_pi = 2.0 * FFA_PI;
+    const double P = two_pi / (double)m.nsectors;
+    double fi = fmod(t, two_pi);
+    if (fi < 0.0) fi += two_pi;
+    double fim = fmod(fi, P);
+    if (fim < 0.0) fim += P;
+    const double r = x[0], pr = x[1], z = x[2], pz = x[3];
+    double Bz, Br, Bfi;
+    order0_f64(m, r, fim, z, Bz, Br, Bfi);
+    if (m.n_harm > 0) harmonics_f64(m, r, fi, Bz);
+    const double pf = sqrt(1.0 - pr * pr - pz * pz);
+    d[0] = r * pr / pf;
+    d[1] = pf - qp * (r * Bz - r * pz / pf * Bfi);
+    d[2] = r * pz / pf;
+    d[3] = qp * (r * Br - r * pr / pf * Bfi);
+}
+
+// rk4_step (src/FFAG_track.py:33-40) repeated on the linspace grid of
+// rk4_solve; one particle per thread, the trajectory optionally recorded
+// component-major so each store is coalesced.
+__global__ __launch_bounds__(BLOCK) void k_orbit_rk4(const ffa_machine *__restrict__ mp, ffa_orbit o) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= o.n) return;
+    const ffa_machine &m = *mp;
+    const int64_t n = o.n;
+    double x[4] = {o.state[i], o.state[n + i], o.state[2 * n + i], o.state[3 * n + i]};
+    const double qp = o.q_over_p[i], h = o.h;
+    if (o.traj)
+        for (int c = 0; c < 4; ++c) o.traj[c * n + i] = x[c];
+    for (int s = 0; s < o.nsteps; ++s) {
+        const double t = (double)s * o.t_step + o.t0;
+        double k1[4], k2[4], k3[4], k4[4], y[4], d[4];
+        seo_rhs(m, qp, t, x, d);
+        for (int c = 0; c < 4; ++c) { k1[c] = h * d[c]; y[c] = x[c] + 0.5 * k1[c]; }
+        seo_rhs(m, qp, t + 0.5 * h, y, d);
+        for (int c = 0; c < 4; ++c) { k2[c] = h * d[c]; y[c] = x[c] + 0.5 * k2[c]; }
+        seo_rhs(m, qp, t + 0.5 * h, y, d);
+        for (int c = 0; c < 4; ++c) { k3[c] = h * d[c]; y[c] = x[c] + k3[c]; }
+        seo_rhs(m, qp, t + h, y, d);
+        for (int c = 0; c < 4; ++c) {
+            k4[c] = h * d[c];
+            x[c] = x[c] + (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]) / 6.0;
+        }
+        if (o.traj)
+            for (int c = 0; c < 4; ++c) o.traj[((size_t)(s + 1) * 4 + c) * n + i] = x[c];
+    }
+    for (int c = 0; c < 4; ++c) o.state[c * n + i] = x[c];
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
+
+hipError_t launch_orbit_rk4(ffa_ctx_s *c, const ffa_orbit *o) {
+    if (o->n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_orbit_rk4, dim3(grid_for(o->n)), dim3(BLOCK), 0, c->stream, ctx_machine_dev(c), *o);
+    return hipGetLastError();
+}
 
 hipError_t launch_inject(ffa_ctx_s *c, const ffa_particles *p, int64_t b, int64_t e) {
     if (e <= b) return hipSuccess;
