@@ -380,6 +380,31 @@ typedef struct ffa_orbit {
 
 int ffa_orbit_track(void *ctx, const ffa_orbit *o);
 
+/* ---- Spiral RF-gap field maps -------------------------------------------
+ * Replaces spiral_distance_direction_rphi_with_flag, the Gaussian gap
+ * profile, project_to_polar_components and the div E = 0 step of
+ * build_spiral_map_from_config (examples/main_Emap.py:267-445, 865-931).
+ * Node (i, j) is (r_axis[i], phi_axis[j]) (continuous azimuth); outputs are
+ * row-major [nr][nphi].  The centre line is phi = s_ref + a ln(r / r_ref) on
+ * [r_min, r_max]; flag 1 / 2 marks nodes whose nearest line point is the
+ * r_min / r_max end (no field there). */
+typedef struct ffa_spiral_gap {
+    const double *r_axis;      /* device [nr], uniform step dr */
+    const double *phi_axis;    /* device [nphi], uniform step dphi */
+    int32_t nr, nphi;
+    double dr, dphi;
+    double r_ref, s_ref, a;    /* a = tan(spiral angle) */
+    double r_min, r_max;
+    double U, delta;           /* profile amplitude and Gaussian sigma (FWHM / 2.35482) */
+    double side;               /* +1 left normal, -1 right normal */
+    int32_t n_iter;            /* Newton iterations (>= 0) */
+    int32_t pad;
+    double *er, *ephi, *ez_over_z, *dist;   /* device outputs [nr][nphi] */
+    int32_t *flag;                          /* device output [nr][nphi] */
+} ffa_spiral_gap;
+
+int ffa_spiral_gap_maps(void *ctx, const ffa_spiral_gap *g);
+
 /* ---- Taylor-coefficient map generator ----------------------------------
  * Replaces the per-radius evaluation loop of Generate_Bmaps_MultiProcess /
  * compute_taylor_coefficients_group (src/FFAG_HighOrder.py:913-998, driven

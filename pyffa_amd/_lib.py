@@ -120,6 +120,13 @@ class FfaOrbit(ctypes.Structure):
                 ("nsteps", _i32), ("pad", _i32), ("traj", _vp)]
 
 
+class FfaSpiralGap(ctypes.Structure):
+    _fields_ = [("r_axis", _vp), ("phi_axis", _vp), ("nr", _i32), ("nphi", _i32), ("dr", _d), ("dphi", _d),
+                ("r_ref", _d), ("s_ref", _d), ("a", _d), ("r_min", _d), ("r_max", _d), ("U", _d), ("delta", _d),
+                ("side", _d), ("n_iter", _i32), ("pad", _i32), ("er", _vp), ("ephi", _vp), ("ez_over_z", _vp),
+                ("dist", _vp), ("flag", _vp)]
+
+
 FFA_TAYLOR_MAX_ORDERS = 8
 FFA_TAYLOR_MAX_POLES = 16
 
@@ -137,7 +144,7 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
             "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing",
-            "ffa_taylor_maps", "ffa_orbit_track")
+            "ffa_taylor_maps", "ffa_orbit_track", "ffa_spiral_gap_maps")
 
 
 def load():
@@ -180,6 +187,7 @@ def load():
                                        _vp, _vp, _vp]
     lib.ffa_taylor_maps.argtypes = [_vp, ctypes.POINTER(FfaTaylorGen)]
     lib.ffa_orbit_track.argtypes = [_vp, ctypes.POINTER(FfaOrbit)]
+    lib.ffa_spiral_gap_maps.argtypes = [_vp, ctypes.POINTER(FfaSpiralGap)]
     for name in EXPORTED:
         if name != "ffa_last_error":
             getattr(lib, name).restype = ctypes.c_int

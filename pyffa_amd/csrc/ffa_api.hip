@@ -268,6 +268,21 @@ int ffa_orbit_track(void *ctx, const ffa_orbit *o) {
     return hip_check(c, launch_orbit_rk4(c, o), "ffa_orbit_track");
 }
 
+int ffa_spiral_gap_maps(void *ctx, const ffa_spiral_gap *g) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !g) return fail(c, FFA_ERR_INVALID, "ctx or spiral gap description is NULL");
+    if (g->nr < 2 || g->nphi < 2) return fail(c, FFA_ERR_INVALID, "the map needs at least 2 x 2 nodes");
+    if ((int64_t)g->nr * g->nphi > ((int64_t)1 << 31) * 64) return fail(c, FFA_ERR_INVALID, "map too large");
+    if (!(g->dr > 0.0) || !(g->dphi > 0.0)) return fail(c, FFA_ERR_INVALID, "dr and dphi must be positive");
+    if (!(g->r_ref > 0.0) || !(g->r_min > 0.0) || !(g->r_max > g->r_min))
+        return fail(c, FFA_ERR_INVALID, "need 0 < r_min < r_max and r_ref > 0");
+    if (!(g->delta > 0.0)) return fail(c, FFA_ERR_INVALID, "delta must be positive");
+    if (g->n_iter < 0) return fail(c, FFA_ERR_INVALID, "n_iter must be >= 0");
+    if (!g->r_axis || !g->phi_axis || !g->er || !g->ephi || !g->ez_over_z || !g->dist || !g->flag)
+        return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    return hip_check(c, launch_spiral_gap(c, g), "ffa_spiral_gap_maps");
+}
+
 int ffa_taylor_maps(void *ctx, const ffa_taylor_gen *g) {
     ffa_ctx_s *c = as_ctx(ctx);
     if (!c || !g) return fail(c, FFA_ERR_INVALID, "ctx or generator description is NULL");

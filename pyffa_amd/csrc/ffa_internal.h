@@ -107,6 +107,7 @@ hipError_t launch_detect_crossing(ffa_ctx_s *c, const double *fi_pre, const uint
                                   int64_t n, double target, int64_t *idx_out, int64_t *count);
 hipError_t launch_taylor_maps(ffa_ctx_s *c, const ffa_taylor_gen *g);
 hipError_t launch_orbit_rk4(ffa_ctx_s *c, const ffa_orbit *o);
+hipError_t launch_spiral_gap(ffa_ctx_s *c, const ffa_spiral_gap *g);
 size_t sort_workspace_bytes(int64_t count);
 hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
                              const uint32_t **order_out);

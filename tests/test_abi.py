@@ -87,7 +87,8 @@ def test_missing_gpu_fails_loudly():
 STRUCTS = [("FfaParticles", "ffa_particles"), ("FfaMachine", "ffa_machine"), ("FfaStep", "ffa_step"),
            ("FfaScParams", "ffa_sc_params"), ("FfaScResult", "ffa_sc_result"), ("FfaScLayout", "ffa_sc_layout"),
            ("FfaMlpDesc", "ffa_mlp_desc"), ("FfaAdam", "ffa_adam"), ("FfaMlpLayout", "ffa_mlp_layout"),
-           ("FfaTaylorGen", "ffa_taylor_gen"), ("FfaOrbit", "ffa_orbit")]
+           ("FfaTaylorGen", "ffa_taylor_gen"), ("FfaOrbit", "ffa_orbit"),
+           ("FfaSpiralGap", "ffa_spiral_gap")]
 
 
 def test_ctypes_mirrors_match_the_compiled_header(tmp_path):
