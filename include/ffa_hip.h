@@ -159,7 +159,7 @@ typedef struct {
     size_t result;      /* ffa_sc_result */
     size_t records;     /* uint64 n: slice-sorted packed CIC records */
     size_t slice_start; /* int32 ns + 1: record offset of each slice */
-    size_t cursor;      /* int32 ns: fill cursors */
+    size_t block_offset; /* int32 [FFA_SC_BLOCKS][ns]: record offset of each block's run inside each slice */
     size_t records_by_row;  /* uint64 n: records of each slice ordered by CIC row */
     size_t row_start;   /* int32 ns * (nz + 1): record offset of each (slice, row) */
     size_t tables;      /* FFT twiddles (4 x 1024 complex64) and the Green's function (nx * nz float32) */

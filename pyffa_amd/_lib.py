@@ -82,7 +82,7 @@ class FfaScResult(ctypes.Structure):
 class FfaScLayout(ctypes.Structure):
     _fields_ = [(name, ctypes.c_size_t) for name in (
         "total", "grid", "cell_count", "cbuf", "phi", "exz", "px", "ppr0", "pslice",
-        "slice_count", "partials", "result", "records", "slice_start", "cursor", "records_by_row", "row_start",
+        "slice_count", "partials", "result", "records", "slice_start", "block_offset", "records_by_row", "row_start",
         "tables")]
 
 

@@ -1,18 +1,20 @@
 // 2.5D space-charge solve of the tracking hot path (gfx950).
 //
 // Pipeline of one solve (every kernel on the context stream, no host sync):
-//   k_sc_fill          zero the per-slice counters
 //   k_sc_stage1        per survivor: gamma, phi mod 2pi; per-block partial
 //                      sums of gamma and the four covering-arc min/max
 //   k_sc_final1        1 block: covering arc, slice step, mean gamma,
-//                      perimeter, bunch length (device-side scalars)
-//   k_sc_stage2        per survivor: SEO r0/pr0, x = r - r0, slice index,
+//                      perimeter, bunch length (device-side scalars); zeroes
+//                      the per-slice counters
+//   k_sc_locate_moments per survivor: SEO r0/pr0, x = r - r0, slice index,
 //                      per-block partial moments, slice histogram
 //   k_sc_final2        1 block: grid box, dx, dz, charge scale, slice offsets
 //   k_sc_bin           per survivor: CIC cell + fixed-point weights packed in
 //                      one 64-bit record, written slice-sorted (two-level
 //                      reservation: one global atomic per block and slice)
-//   k_sc_rowsort       per slice: records ordered by CIC row, row offsets
+//   k_sc_rowsort       per slice: records ordered by CIC row, row offsets;
+//                      the blocks also fill the per-solve FFT twiddles and
+//                      Green's function table
 //   k_sc_deposit_tiles one workgroup per (slice, z band): LDS int64 / int32
 //                      accumulation of the slice's records, plain stores of
 //                      the band (integer sums: bit-identical for any order,
@@ -90,10 +92,6 @@ __device__ __forceinline__ int32_t wave_counter_add(int32_t *ctr, int key, bool 
     return out;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sc_fill(int32_t *slc, int ns) {
-    for (int s = threadIdx.x; s < ns; s += BLOCK) slc[s] = 0;
-}
-
 // GetLocalCoordinates_2p5(_rk) + divide_arc2's scan (src/FFAG_SC_new.py:74-148, 534-566).
 __global__ __launch_bounds__(BLOCK) void k_sc_stage1(ffa_particles p, int rk, double c, double E0J,
                                                     double *part) {
@@ -159,8 +157,10 @@ __device__ double np_interp(double x, const double *xp, const double *fp, int n)
 // Covering arc (divide_arc2 + merge_multiple_arcs for one rank,
 // src/FFAG_SC_new.py:534-606; src/FFAG_Bunch.py:1203-1215), mean gamma and
 // bunch length (src/FFAG_Bunch.py:1217-1294).
-__global__ __launch_bounds__(BLOCK) void k_sc_final1(const double *part, ffa_sc_params sp, ffa_sc_result *res) {
+__global__ __launch_bounds__(BLOCK) void k_sc_final1(const double *part, ffa_sc_params sp, ffa_sc_result *res,
+                                                    int32_t *slc) {
     __shared__ double sh[10][BLOCK];
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) slc[k] = 0;
     double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
     const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
     for (int b = threadIdx.x; b < NB; b += BLOCK) {
@@ -210,46 +210,25 @@ __device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, 
     return (1 - tx) * (1 - ty) * z00 + (1 - tx) * ty * z01 + tx * (1 - ty) * z10 + tx * ty * z11;
 }
 
-// SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531)
-// in two kernels: k_sc_locate per particle, k_sc_moments for the sums and
-// the slice histogram.
+// SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
 constexpr int MAX_NS_LDS = 4096;
 
-// Per particle (one per thread, so the SEO table reads of many particles are
-// in flight at once): x = r - r0(Ek, phi), pr0 and the slice index.
-__global__ __launch_bounds__(BLOCK) void k_sc_locate(ffa_particles p, ffa_sc_params sp, double c, double E0J,
-                                                    const ffa_sc_result *res, double *px, double *ppr0,
-                                                    int32_t *pslice) {
-    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-    if (i >= p.n) return;
-    if (!(p.flags[i] & FFA_F_ALIVE)) { pslice[i] = -1; return; }
-    const double fs = res->fi_s, step = res->slice_step;
-    const double g = gamma_of(p, i, sp.rk, c, E0J);
-    const double fm = mod2pi(p.fi[i]);
-    const double ek = (g - 1) * sp.E0_MeV;
-    const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-    const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-    double f = fm;
-    if (f < fs) f += FFA_TWO_PI;
-    double fsl = (f - fs) / step;
-    fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
-    int64_t s = (int64_t)fsl;
-    if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-    px[i] = p.r[i] - r0;
-    ppr0[i] = pr0;
-    pslice[i] = (int32_t)s;
-}
-
-// Moments of the survivors (sum x, z, x^2, z^2) and the slice histogram:
-// per-block counts in LDS, one global atomic per non-empty (block, slice)
-// pair (ns <= FFA_SC_MAX_NS).  Fixed block partition, so the sums are
-// reproducible run to run.
-__global__ __launch_bounds__(BLOCK) void k_sc_moments(ffa_particles p, ffa_sc_params sp, const double *px,
-                                                     const int32_t *pslice, int32_t *slc, double *part) {
+// Per survivor: x = r - r0(Ek, phi), pr0 and the slice index (written for
+// the bin and gather kernels), the moments (sum x, z, x^2, z^2) and the
+// slice histogram: per-block counts in LDS, one global atomic per non-empty
+// (block, slice) pair (ns <= FFA_SC_MAX_NS).  Fixed block partition and
+// per-thread order, so the sums are reproducible run to run; UNR particles
+// per pass have their loads (state, then the SEO table corners) in flight
+// together.
+__global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ffa_sc_params sp, double c,
+                                                            double E0J, const ffa_sc_result *res, double *px,
+                                                            double *ppr0, int32_t *pslice, int32_t *slc,
+                                                            int32_t *blk_off, double *part) {
     __shared__ double sh[4][BLOCK];
     __shared__ int32_t hist[MAX_NS_LDS];
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
     __syncthreads();
+    const double fs = res->fi_s, step = res->slice_step;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t S = (int64_t)NB * BLOCK;
     for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
@@ -258,10 +237,26 @@ __global__ __launch_bounds__(BLOCK) void k_sc_moments(ffa_particles p, ffa_sc_pa
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const int64_t i = i0 + u * S;
-            s[u] = i < p.n ? pslice[i] : -1;
-            const int64_t j = s[u] >= 0 ? i : 0;
-            x[u] = s[u] >= 0 ? px[j] : 0.0;
-            z[u] = s[u] >= 0 ? p.z[j] : 0.0;
+            const bool ok = i < p.n && (p.flags[i < p.n ? i : 0] & FFA_F_ALIVE);
+            const int64_t j = ok ? i : 0;
+            const double g = gamma_of(p, j, sp.rk, c, E0J);
+            const double fm = mod2pi(p.fi[j]);
+            const double ek = (g - 1) * sp.E0_MeV;
+            const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+            const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
+            double f = fm;
+            if (f < fs) f += FFA_TWO_PI;
+            double fsl = (f - fs) / step;
+            fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
+            int64_t sl = (int64_t)fsl;
+            if (sl < 0) sl = 0; else if (sl >= sp.ns) sl = sp.ns - 1;
+            s[u] = ok ? (int)sl : -1;
+            x[u] = p.r[j] - r0;
+            z[u] = p.z[j];
+            if (i < p.n) {
+                pslice[i] = s[u];
+                if (ok) { px[i] = x[u]; ppr0[i] = pr0; }
+            }
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
@@ -271,15 +266,19 @@ __global__ __launch_bounds__(BLOCK) void k_sc_moments(ffa_particles p, ffa_sc_pa
     }
     const int op[4] = {0, 0, 0, 0};
     block_reduce<4>(sh, v, op);       // contains __syncthreads: hist is complete
+    // the slice counter's value before this block's add is where the block's
+    // run of records starts inside the slice (entries of slices the block
+    // does not hold stay unwritten and are never read)
+    int32_t *bo = blk_off + (size_t)blockIdx.x * sp.ns;
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK)
-        if (hist[k]) atomicAdd(&slc[k], hist[k]);
+        if (hist[k]) bo[k] = atomicAdd(&slc[k], hist[k]);
     if (threadIdx.x == 0)
         for (int k = 0; k < 4; ++k) part[16 * blockIdx.x + 10 + k] = v[k];
 }
 
 // Grid box from the moments (src/FFAG_Bunch.py:1259-1322).
 __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res,
-                                                    const int32_t *slc, int32_t *slice_start, int32_t *cursor) {
+                                                    const int32_t *slc, int32_t *slice_start) {
     __shared__ double sh[4][BLOCK];
     __shared__ int32_t tot[BLOCK];
     {   // exclusive scan of the slice counts (ns <= 4096): each thread owns a
@@ -299,7 +298,6 @@ __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_
         int32_t acc = tot[threadIdx.x] - own;
         for (int k = k0; k < k1; ++k) {
             slice_start[k] = acc;
-            cursor[k] = acc;
             acc += slc[k];
         }
         if (threadIdx.x == BLOCK - 1) slice_start[sp.ns] = tot[BLOCK - 1];
@@ -350,31 +348,22 @@ __device__ __forceinline__ uint64_t cic_record(double x, double z, const ffa_sc_
     return (uint64_t)ix | ((uint64_t)iz << 10) | (a << 20) | (b << 41);
 }
 
-// Bin the survivors into slice-sorted records.  Same block partition as
-// k_sc_stage2; pass 1 counts the block's particles per slice in LDS, one
-// global atomic per non-empty (block, slice) reserves a contiguous range,
-// pass 2 places each record with an LDS rank.
+// Bin the survivors into slice-sorted records.  Same block partition and
+// per-thread order as k_sc_locate_moments, which counted each block's
+// particles per slice and reserved the block's run inside each slice: the
+// LDS counters start at the runs' global offsets and each record takes the
+// next place of its slice's run.
 __global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
-                                                 const double *px, const int32_t *pslice, int32_t *cursor,
-                                                 uint64_t *rec) {
-    __shared__ int32_t hist[MAX_NS_LDS];
+                                                 const double *px, const int32_t *pslice, const int32_t *slice_start,
+                                                 const int32_t *blk_off, uint64_t *rec) {
+    __shared__ int32_t place[MAX_NS_LDS];
     if (res->bad_grid != 0.0) return;
-    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
+    const int32_t *bo = blk_off + (size_t)blockIdx.x * sp.ns;
+    // entries of slices the block holds no particle of are stale; unsigned
+    // sums keep those (never used) entries well defined
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) place[k] = (int32_t)((uint32_t)slice_start[k] + (uint32_t)bo[k]);
     __syncthreads();
     const int64_t S = (int64_t)NB * BLOCK;
-    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
-        int s[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) s[u] = i0 + u * S < p.n ? pslice[i0 + u * S] : -1;
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) {
-        const int c = hist[k];
-        hist[k] = c ? atomicAdd(&cursor[k], c) : 0;      // base of this block's range
-    }
-    __syncthreads();
     for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
         int s[UNR];
         double x[UNR], z[UNR];
@@ -388,10 +377,43 @@ __global__ __launch_bounds__(BLOCK) void k_sc_bin(ffa_particles p, ffa_sc_params
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const int pos = wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
+            const int pos = wave_counter_add(place, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
             if (s[u] >= 0) rec[pos] = cic_record(x[u], z[u], res, sp.nx, sp.nz, sp.bits);
         }
     }
+}
+
+// Per-solve tables (tables region): twiddles w^k for k < N of the x and z
+// transforms, forward and inverse, at [0..4) * 1024, then the periodic
+// Green's function (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760)
+// in the reference's float32 arithmetic, transposed to [ci][j] so a column
+// group reads one contiguous run.
+constexpr int TW_STRIDE = 1024;
+
+__device__ __forceinline__ void sc_table_entry(int64_t t, float2 *tw, float *green, const ffa_sc_params &sp,
+                                               const ffa_sc_result *res) {
+    const int nx = sp.nx, nz = sp.nz;
+    if (t < 4 * TW_STRIDE) {
+        const int which = (int)(t / TW_STRIDE), k = (int)(t - which * TW_STRIDE);
+        const int N = which < 2 ? nx : nz;
+        if (k < N) {
+            double s, c;
+            sincospi(-2.0 * (double)k / (double)N, &s, &c);
+            tw[t] = make_float2((float)c, (float)((which & 1) ? -s : s));
+        }
+        return;
+    }
+    const int64_t g = t - 4 * TW_STRIDE;
+    if (g >= (int64_t)nx * nz) return;
+    const int ci = (int)(g / nz), j = (int)(g - (int64_t)ci * nz);
+    const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
+    const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
+    const int idx = ci <= nx / 2 ? ci : ci - nx;
+    const int idz = j <= nz / 2 ? j : j - nz;
+    const float kx = (float)((double)dkx * idx), kz = (float)((double)dkz * idz);
+    const float kz2 = kz * kz;
+    const float k2 = kx * kx + kz2;
+    green[g] = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
 }
 
 // One workgroup per slice: order the slice's records by CIC row iz (LDS
@@ -402,9 +424,15 @@ constexpr int ROW_BLOCK = 1024;
 
 __global__ __launch_bounds__(ROW_BLOCK) void k_sc_rowsort(ffa_sc_params sp, const ffa_sc_result *res,
                                                          const int32_t *slice_start, const uint64_t *rec,
-                                                         uint64_t *rec2, int32_t *row_start) {
+                                                         uint64_t *rec2, int32_t *row_start, float2 *tw,
+                                                         float *green) {
     __shared__ int32_t cnt[1024];
     __shared__ int32_t tot[ROW_BLOCK];
+    {   // this block's share of the per-solve tables (read by the FFT kernels)
+        const int64_t n_tab = 4 * TW_STRIDE + (int64_t)sp.nx * sp.nz;
+        for (int64_t t = blockIdx.x * (int64_t)ROW_BLOCK + threadIdx.x; t < n_tab; t += (int64_t)gridDim.x * ROW_BLOCK)
+            sc_table_entry(t, tw, green, sp, res);
+    }
     const int nz = sp.nz, s = blockIdx.x;
     int32_t *rs = row_start + (size_t)s * (nz + 1);
     const int r0 = slice_start[s], r1 = slice_start[s + 1];
@@ -496,40 +524,6 @@ constexpr int FFT_PTS = 1024;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// Per-solve tables (tables region): twiddles w^k for k < N of the x and z
-// transforms, forward and inverse, at [0..4) * 1024, then the periodic
-// Green's function (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760)
-// in the reference's float32 arithmetic, transposed to [ci][j] so a column
-// group reads one contiguous run.
-constexpr int TW_STRIDE = 1024;
-
-__global__ __launch_bounds__(BLOCK) void k_sc_tables(float2 *tw, float *green, ffa_sc_params sp,
-                                                    const ffa_sc_result *res) {
-    const int nx = sp.nx, nz = sp.nz;
-    const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-    if (t < 4 * TW_STRIDE) {
-        const int which = (int)(t / TW_STRIDE), k = (int)(t - which * TW_STRIDE);
-        const int N = which < 2 ? nx : nz;
-        if (k < N) {
-            double s, c;
-            sincospi(-2.0 * (double)k / (double)N, &s, &c);
-            tw[t] = make_float2((float)c, (float)((which & 1) ? -s : s));
-        }
-        return;
-    }
-    const int64_t g = t - 4 * TW_STRIDE;
-    if (g >= (int64_t)nx * nz) return;
-    const int ci = (int)(g / nz), j = (int)(g - (int64_t)ci * nz);
-    const float dkx = (float)(2.0 * FFA_PI / (nx * res->dx));
-    const float dkz = (float)(2.0 * FFA_PI / (nz * res->dz));
-    const int idx = ci <= nx / 2 ? ci : ci - nx;
-    const int idz = j <= nz / 2 ? j : j - nz;
-    const float kx = (float)((double)dkx * idx), kz = (float)((double)dkz * idz);
-    const float kz2 = kz * kz;
-    const float k2 = kx * kx + kz2;
-    green[g] = k2 == 0.0f ? 0.0f : (float)(1.0 / (EPS0 * (double)k2));
 }
 
 __device__ __forceinline__ void load_twiddles(float2 *tw, const float2 *tab, int N) {
@@ -841,7 +835,7 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     L->result = o; o = al(o + sizeof(ffa_sc_result));
     L->records = o; o = al(o + 8 * (size_t)n);
     L->slice_start = o; o = al(o + 4 * ((size_t)ns + 1));
-    L->cursor = o; o = al(o + 4 * (size_t)ns);
+    L->block_offset = o; o = al(o + 4 * (size_t)NB * ns);
     L->records_by_row = o; o = al(o + 8 * (size_t)n);
     L->row_start = o; o = al(o + 4 * (size_t)ns * ((size_t)nz + 1));
     L->tables = o; o = al(o + 8 * 4 * (size_t)TW_STRIDE + 4 * (size_t)nz * nx);
@@ -869,29 +863,24 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     hipStream_t st = c->stream;
     uint64_t *rec = reinterpret_cast<uint64_t *>(w + L->records);
     int32_t *sstart = reinterpret_cast<int32_t *>(w + L->slice_start);
-    int32_t *cursor = reinterpret_cast<int32_t *>(w + L->cursor);
-    hipLaunchKernelGGL(k_sc_fill, dim3(1), dim3(BLOCK), 0, st, slc, sp->ns);
+    int32_t *blk_off = reinterpret_cast<int32_t *>(w + L->block_offset);
     hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, sp->c, E0J, part);
-    hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res);
-    if (p->n > 0)
-        hipLaunchKernelGGL(k_sc_locate, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0,
-                           pslice);
-    hipLaunchKernelGGL(k_sc_moments, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, px, pslice, slc, part);
-    hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart, cursor);
-    hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, cursor, rec);
+    hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc);
+    hipLaunchKernelGGL(k_sc_locate_moments, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0, pslice,
+                       slc, blk_off, part);
+    hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart);
+    hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, sstart, blk_off, rec);
     const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
     const size_t tile_lds = (size_t)bz * sp->nx * (sp->cell_counts ? 12 : 8);
     uint64_t *rec2 = reinterpret_cast<uint64_t *>(w + L->records_by_row);
     int32_t *rstart = reinterpret_cast<int32_t *>(w + L->row_start);
+    float2 *twt = reinterpret_cast<float2 *>(w + L->tables);
+    float *green = reinterpret_cast<float *>(w + L->tables + 8 * 4 * (size_t)TW_STRIDE);
     hipLaunchKernelGGL(k_sc_rowsort, dim3((unsigned)sp->ns), dim3(ROW_BLOCK), 0, st, *sp, res, sstart, rec, rec2,
-                       rstart);
+                       rstart, twt, green);
     hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st, *sp,
                        res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
     const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
-    float2 *twt = reinterpret_cast<float2 *>(w + L->tables);
-    float *green = reinterpret_cast<float *>(w + L->tables + 8 * 4 * (size_t)TW_STRIDE);
-    hipLaunchKernelGGL(k_sc_tables, dim3(gridn(4 * TW_STRIDE + (int64_t)sp->nx * sp->nz)), dim3(BLOCK), 0, st, twt, green,
-                       *sp, res);
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
     const size_t rows_lds = sizeof(float2) * (2 * FFT_PTS + sp->nx);

@@ -59,10 +59,11 @@ def test_mlp_workspace_layout():
 def test_workspace_layout_is_aligned_and_ordered():
     L = _lib.sc_layout(1000, 33, 64, 32)
     offs = [L.grid, L.cell_count, L.cbuf, L.phi, L.exz, L.px, L.ppr0, L.pslice, L.slice_count, L.partials,
-            L.result, L.records, L.slice_start, L.cursor, L.records_by_row, L.row_start, L.tables]
+            L.result, L.records, L.slice_start, L.block_offset, L.records_by_row, L.row_start, L.tables]
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs) and L.total > offs[-1]
     assert L.cell_count - L.grid >= 8 * 33 * 64 * 32
+    assert L.records_by_row - L.block_offset >= 4 * _lib.FFA_SC_BLOCKS * 33
 
 
 def test_invalid_grid_is_rejected_without_gpu():
