@@ -222,8 +222,12 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
 // all 64 lanes.
 constexpr int SUR_BLOCK = 512;
 
+// Without the surrogate the kernel is held to 96 VGPRs (5 waves per SIMD):
+// the extra wave hides more of the gathers' and constants' latency than the
+// few spilled registers cost.
 template <typename ET, bool SUR>
-__global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
+__global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK)
+__attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
                                                       const ffa_fconst *__restrict__ kc,
                                                       const ffa_step *__restrict__ steps,
                                                       const int64_t *__restrict__ inj, int nsteps,

@@ -22,8 +22,11 @@
 //   k_fft_rows_fwd     int64 -> float, two slices packed as re/im, row FFTs
 //   k_fft_cols_green   column FFT, x periodic Green's function, inverse
 //   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi
-//   k_sc_field         -grad(phi)/gamma^2 with the reference's edge stencils
-//   k_sc_gather_roll   bilinear gather + pr0 projection, endpoint roll
+//   k_sc_field         -grad(phi)/gamma^2 with the reference's edge stencils,
+//                      stored as (Ex, Ez) cubes (only when sp.field_cubes)
+//   k_sc_gather_roll   bilinear gather + pr0 projection, endpoint roll; without
+//                      the cubes it applies the same stencils to phi at the
+//                      particle's four cells
 // Particle-side arithmetic is FP64 in the reference's operation order (this
 // unit is compiled with -ffp-contract=off); the grid is float32 like the
 // reference's (complex64 FFTs).  Reductions use a fixed block partition and
@@ -157,10 +160,21 @@ __device__ double np_interp(double x, const double *xp, const double *fp, int n)
 // Covering arc (divide_arc2 + merge_multiple_arcs for one rank,
 // src/FFAG_SC_new.py:534-606; src/FFAG_Bunch.py:1203-1215), mean gamma and
 // bunch length (src/FFAG_Bunch.py:1217-1294).
+constexpr int EK_LDS = 1024;
+
 __global__ __launch_bounds__(BLOCK) void k_sc_final1(const double *part, ffa_sc_params sp, ffa_sc_result *res,
                                                     int32_t *slc) {
     __shared__ double sh[10][BLOCK];
+    __shared__ double ek_ax[EK_LDS], ek_per[EK_LDS];
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK) slc[k] = 0;
+    // the perimeter table is searched by one thread: stage it in LDS so the
+    // binary search does not wait on a global load per probe
+    const bool stage = sp.n_ek <= EK_LDS;
+    if (stage)
+        for (int k = threadIdx.x; k < sp.n_ek; k += BLOCK) {
+            ek_ax[k] = sp.seo_ek_axis[k];
+            ek_per[k] = sp.seo_perimeter[k];
+        }
     double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
     const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
     for (int b = threadIdx.x; b < NB; b += BLOCK) {
@@ -183,7 +197,8 @@ __global__ __launch_bounds__(BLOCK) void k_sc_final1(const double *part, ffa_sc_
     if (fe <= fs) fe += 2.0 * FFA_PI;
     const double cnt = v[1];
     const double mgm1 = cnt > 0.0 ? v[0] / cnt - 1.0 : 0.0;
-    const double perim = np_interp(mgm1 * sp.E0_MeV, sp.seo_ek_axis, sp.seo_perimeter, sp.n_ek);
+    const double perim = stage ? np_interp(mgm1 * sp.E0_MeV, ek_ax, ek_per, sp.n_ek)
+                               : np_interp(mgm1 * sp.E0_MeV, sp.seo_ek_axis, sp.seo_perimeter, sp.n_ek);
     res->count = cnt;
     res->sum_gamma = v[0];
     for (int k = 0; k < 4; ++k) { res->arc_min[k] = v[2 + k]; res->arc_max[k] = v[6 + k]; }
@@ -232,30 +247,62 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t S = (int64_t)NB * BLOCK;
     for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
-        int s[UNR];
-        double x[UNR], z[UNR];
+        // phases over the UNR particles (flags; state; SEO corners; results),
+        // all loads ahead of the stores so the particles' memory round trips
+        // overlap
+        bool ok[UNR];
+        int64_t jj[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const int64_t i = i0 + u * S;
-            const bool ok = i < p.n && (p.flags[i < p.n ? i : 0] & FFA_F_ALIVE);
-            const int64_t j = ok ? i : 0;
-            const double g = gamma_of(p, j, sp.rk, c, E0J);
-            const double fm = mod2pi(p.fi[j]);
-            const double ek = (g - 1) * sp.E0_MeV;
-            const double r0 = seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-            const double pr0 = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek, fm);
-            double f = fm;
+            const uint8_t f = p.flags[i < p.n ? i : 0];
+            ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
+            jj[u] = ok[u] ? i : 0;
+        }
+        double ek[UNR], fm[UNR], x[UNR], z[UNR], fd[UNR], vr[UNR], vz[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            x[u] = p.r[jj[u]];
+            fd[u] = p.fidot[jj[u]];
+            vr[u] = p.vr[jj[u]];
+            vz[u] = p.vz[jj[u]];
+            fm[u] = p.fi[jj[u]];
+            z[u] = p.z[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            // gamma_of's arithmetic on the loaded state
+            double g;
+            if (sp.rk) {
+                g = fd[u] / E0J;
+            } else {
+                const double rd = x[u] * fd[u];
+                const double v2 = vr[u] * vr[u] + vz[u] * vz[u] + rd * rd;
+                g = 1.0 / sqrt(1.0 - v2 / (c * c));
+            }
+            ek[u] = (g - 1) * sp.E0_MeV;
+            fm[u] = mod2pi(fm[u]);
+        }
+        double pr0[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            x[u] -= seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
+            pr0[u] = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
+        }
+        int s[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            double f = fm[u];
             if (f < fs) f += FFA_TWO_PI;
             double fsl = (f - fs) / step;
             fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
             int64_t sl = (int64_t)fsl;
             if (sl < 0) sl = 0; else if (sl >= sp.ns) sl = sp.ns - 1;
-            s[u] = ok ? (int)sl : -1;
-            x[u] = p.r[j] - r0;
-            z[u] = p.z[j];
+            s[u] = ok[u] ? (int)sl : -1;
+            const int64_t i = i0 + u * S;
             if (i < p.n) {
                 pslice[i] = s[u];
-                if (ok) { px[i] = x[u]; ppr0[i] = pr0; }
+                if (ok[u]) { px[i] = x[u]; ppr0[i] = pr0[u]; }
             }
         }
 #pragma unroll
@@ -281,6 +328,9 @@ __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_
                                                     const int32_t *slc, int32_t *slice_start) {
     __shared__ double sh[4][BLOCK];
     __shared__ int32_t tot[BLOCK];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};       // loads issued before the scan's
+    for (int b = threadIdx.x; b < NB; b += BLOCK)
+        for (int k = 0; k < 4; ++k) v[k] += part[16 * b + 10 + k];
     {   // exclusive scan of the slice counts (ns <= 4096): each thread owns a
         // contiguous run of ceil(ns / BLOCK) slices
         const int per = (sp.ns + BLOCK - 1) / BLOCK;
@@ -302,9 +352,6 @@ __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_
         }
         if (threadIdx.x == BLOCK - 1) slice_start[sp.ns] = tot[BLOCK - 1];
     }
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < NB; b += BLOCK)
-        for (int k = 0; k < 4; ++k) v[k] += part[16 * b + 10 + k];
     const int op[4] = {0, 0, 0, 0};
     block_reduce<4>(sh, v, op);
     if (threadIdx.x != 0) return;
@@ -685,6 +732,29 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, floa
 // (Ex, Ez) interleaved, so the gather reads both with one access per corner.
 constexpr int FIELD_ROWS = 4;
 
+// (Ex, Ez) of cell (j, i) from phi at the cell (pc), above and below it in z
+// (pu, pd: rows clamped to the grid) and left and right of it in x (lf, rt:
+// columns clamped to the row).  Shared by k_sc_field and the gather that
+// differentiates phi itself, so both give the same float32 values.
+__device__ __forceinline__ float2 cell_field(float pc, float pu, float pd, float lf, float rt, int i, int j, int nx,
+                                             int nz, double dr, double dz, double i2r, double i2z, double ig2) {
+    double ex, ez;
+    if (i == 0 || i == nx - 1) {              // one-sided in x; z neighbours clamped
+        ex = i == 0 ? -(double)(rt - pc) / dr * ig2 : -(double)(pc - lf) / dr * ig2;
+        ez = -(double)(pu - pd) / dz * ig2;
+    } else if (j == 0) {
+        ex = -(double)(rt - lf) * i2r * ig2;
+        ez = -(double)(pu - pc) / dz * ig2;
+    } else if (j == nz - 1) {
+        ex = -(double)(rt - lf) * i2r * ig2;
+        ez = -(double)(pc - pd) / dz * ig2;
+    } else {
+        ex = -(double)(rt - lf) * i2r * ig2;
+        ez = -(double)(pu - pd) * i2z * ig2;
+    }
+    return make_float2((float)ex, (float)ez);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *exz, ffa_sc_params sp,
                                                    const ffa_sc_result *res, int lognx) {
     const int nx = sp.nx, nz = sp.nz;
@@ -723,31 +793,41 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
     }
     for (int c = 0; c < FIELD_ROWS; ++c) {
         const int j = j0 + c;
-        const float pc = col[c + 1], pu = col[c + 2], pd = col[c];
-        double ex, ez;
-        if (i == 0 || i == nx - 1) {              // one-sided in x; z neighbours clamped
-            ex = i == 0 ? -(double)(rt[c] - pc) / dr * ig2 : -(double)(pc - lf[c]) / dr * ig2;
-            ez = -(double)(pu - pd) / dz * ig2;
-        } else if (j == 0) {
-            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
-            ez = -(double)(pu - pc) / dz * ig2;
-        } else if (j == nz - 1) {
-            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
-            ez = -(double)(pc - pd) / dz * ig2;
-        } else {
-            ex = -(double)(rt[c] - lf[c]) * i2r * ig2;
-            ez = -(double)(pu - pd) * i2z * ig2;
-        }
-        exz[base + (int64_t)j * nx + i] = make_float2((float)ex, (float)ez);
+        exz[base + (int64_t)j * nx + i] =
+            cell_field(col[c + 1], col[c + 2], col[c], lf[c], rt[c], i, j, nx, nz, dr, dz, i2r, i2z, ig2);
     }
+}
+
+// (Ex, Ez) at the gather's four cells (i0 .. i0 + 1, j0 .. j0 + 1, with
+// 0 <= i0 <= nx - 2 and 0 <= j0 <= nz - 2) straight from phi: the 4 x 4
+// stencil block minus its corners, rows and columns clamped to the grid as
+// k_sc_field clamps them.
+__device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int j0, int nx, int nz, double dr,
+                                                     double dz, double i2r, double i2z, double ig2, float2 &a00,
+                                                     float2 &a10, float2 &a01, float2 &a11) {
+    int64_t row[4];
+    int col[4];
+    for (int k = 0; k < 4; ++k) {
+        const int j = j0 - 1 + k, i = i0 - 1 + k;
+        row[k] = (int64_t)(j < 0 ? 0 : (j > nz - 1 ? nz - 1 : j)) * nx;
+        col[k] = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+    }
+    const float v01 = P[row[0] + col[1]], v02 = P[row[0] + col[2]];
+    const float v10 = P[row[1] + col[0]], v11 = P[row[1] + col[1]], v12 = P[row[1] + col[2]], v13 = P[row[1] + col[3]];
+    const float v20 = P[row[2] + col[0]], v21 = P[row[2] + col[1]], v22 = P[row[2] + col[2]], v23 = P[row[2] + col[3]];
+    const float v31 = P[row[3] + col[1]], v32 = P[row[3] + col[2]];
+    a00 = cell_field(v11, v21, v01, v10, v12, i0, j0, nx, nz, dr, dz, i2r, i2z, ig2);
+    a10 = cell_field(v12, v22, v02, v11, v13, i0 + 1, j0, nx, nz, dr, dz, i2r, i2z, ig2);
+    a01 = cell_field(v21, v31, v11, v20, v22, i0, j0 + 1, nx, nz, dr, dz, i2r, i2z, ig2);
+    a11 = cell_field(v22, v32, v12, v21, v23, i0 + 1, j0 + 1, nx, nz, dr, dz, i2r, i2z, ig2);
 }
 
 // interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
 // fused with the endpoint roll of src/FFAG_track.py:356-382.
-template <typename ET>
+template <typename ET, bool FROM_PHI>
 __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
-                                                         const float2 *exz, const double *px, const double *ppr0,
-                                                         const int32_t *pslice) {
+                                                         const float2 *exz, const float *phi, const double *px,
+                                                         const double *ppr0, const int32_t *pslice) {
     const int64_t i = xcd_block() * (int64_t)BLOCK + threadIdx.x;
     if (i >= p.n) return;
     const int64_t n = p.n;
@@ -765,7 +845,6 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             const int nx = sp.nx, nz = sp.nz;
             int s = pslice[i];
             if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-            const float2 *E = exz + (size_t)s * nz * nx;
             const double t = (px[i] - res->xmin) * (1.0 / res->dx), u = (p.z[i] - res->zmin) * (1.0 / res->dz);
             const double pr = ppr0[i];
             const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
@@ -774,8 +853,16 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
             const double al = t - (double)i0, be = u - (double)j0;
             const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
-            const int64_t b = j0 * nx + i0;
-            const float2 a00 = E[b], a10 = E[b + 1], a01 = E[b + nx], a11 = E[b + nx + 1];
+            float2 a00, a10, a01, a11;
+            if constexpr (FROM_PHI) {
+                const double dr = res->dx, dz = res->dz, gm = res->mean_gamma_m1 + 1.0;
+                cells_field_from_phi(phi + (size_t)s * nz * nx, (int)i0, (int)j0, nx, nz, dr, dz, 0.5 / dr,
+                                     0.5 / dz, 1.0 / (gm * gm), a00, a10, a01, a11);
+            } else {
+                const float2 *E = exz + (size_t)s * nz * nx;
+                const int64_t b = j0 * nx + i0;
+                a00 = E[b]; a10 = E[b + 1]; a01 = E[b + nx]; a11 = E[b + nx + 1];
+            }
             const double er = w00 * a00.x + w10 * a10.x + w01 * a01.x + w11 * a11.x;
             const double ez = w00 * a00.y + w10 * a10.y + w01 * a01.y + w11 * a11.y;
             const double ip = 1.0 / sqrt(1.0 + pr * pr);
@@ -891,15 +978,27 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
                        cols_lds, st, cbuf, *sp, twt, green, lognz);
     hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf, phi,
                        *sp, twt, lognx, lognz);
-    hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)(C / FIELD_ROWS))), dim3(BLOCK), 0, st, phi, exz, *sp, res,
-                       lognx);
+    const bool cubes = sp->field_cubes != 0;
+    if (cubes)
+        hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)(C / FIELD_ROWS))), dim3(BLOCK), 0, st, phi, exz, *sp, res,
+                           lognx);
     if (p->n > 0) {
-        if (p->esc_f32)
-            hipLaunchKernelGGL(k_sc_gather_roll<float>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, exz, px,
-                               ppr0, pslice);
-        else
-            hipLaunchKernelGGL(k_sc_gather_roll<double>, dim3(gridn(p->n)), dim3(BLOCK), 0, st, *p, *sp, res, exz, px,
-                               ppr0, pslice);
+        const dim3 g(gridn(p->n));
+        if (p->esc_f32) {
+            if (cubes)
+                hipLaunchKernelGGL((k_sc_gather_roll<float, false>), g, dim3(BLOCK), 0, st, *p, *sp, res, exz, phi, px,
+                                   ppr0, pslice);
+            else
+                hipLaunchKernelGGL((k_sc_gather_roll<float, true>), g, dim3(BLOCK), 0, st, *p, *sp, res, exz, phi, px,
+                                   ppr0, pslice);
+        } else {
+            if (cubes)
+                hipLaunchKernelGGL((k_sc_gather_roll<double, false>), g, dim3(BLOCK), 0, st, *p, *sp, res, exz, phi, px,
+                                   ppr0, pslice);
+            else
+                hipLaunchKernelGGL((k_sc_gather_roll<double, true>), g, dim3(BLOCK), 0, st, *p, *sp, res, exz, phi, px,
+                                   ppr0, pslice);
+        }
     }
     return hipGetLastError();
 }

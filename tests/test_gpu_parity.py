@@ -131,6 +131,7 @@ def test_sc_solve_parity(torch_cuda, grid):
     B = FFAG_Bunch(M, marcosize=marcosize, sc_grid_size=(ns, nx, nz, 4.0), BunchType="Boris")
     cases.set_bunch_seo(B)
     B.sc_cell_counts = True
+    B.sc_field_cubes = True
     B.sc_solve_device(first=True, s_last=0.0)
     res = B.sc_result()
     seo = cases.oracle_seo()
@@ -169,6 +170,37 @@ def test_sc_solve_parity(torch_cuda, grid):
     assert np.array_equal(B.esc_prepre.cpu().numpy(), B.esc_pre.cpu().numpy())
     fl = B.flags.cpu().numpy()
     assert np.array_equal((fl & 4) != 0, alive) and not np.any(fl & 8)
+
+
+@pytest.mark.parametrize("esc_f32", [False, True])
+def test_sc_kicks_from_phi_match_the_field_cubes(torch_cuda, esc_f32):
+    """The gather that differentiates phi at each particle's four cells gives
+    bit-identical kicks to the gather from the stored (Ex, Ez) cubes, edge
+    cells included (a wide bunch puts particles on the grid's border rows and
+    columns)."""
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch
+    print(f"[case] sc kicks from phi vs cubes esc_f32={esc_f32}", flush=True)
+    n = 20000
+    M = cases.bunch("stable", n=n, seed=5, fi_spread=0.6)
+    M[:, 7] = 1.0
+    M[:, 8] = 1.0
+    M[::89, 8] = 0.0
+    out = []
+    for cubes in (True, False):
+        B = FFAG_Bunch(M.copy(), marcosize=1e13 / n, sc_grid_size=(32, 64, 32, 1.5), BunchType="Boris",
+                       precision="fp32" if esc_f32 else "fp64")
+        cases.set_bunch_seo(B)
+        B.sc_field_cubes = cubes
+        B.sc_solve_device(first=True, s_last=0.0)
+        out.append((B.esc_pre.cpu().numpy().copy(), B.flags.cpu().numpy().copy()))
+        if cubes:   # sigma_ratio 1.5: many particles sit outside the box, on the border cells
+            px = B.sc_view("px").cpu().numpy()
+            res = B.sc_result()
+            alive = M[:, 8] != 0
+            assert np.mean((px[alive] < res["xmin"] + res["dx"]) | (px[alive] > res["xmax"] - res["dx"])) > 0.01
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.count_nonzero(out[0][0]) > 0
 
 
 def test_track_with_sc_parity(torch_cuda, tmp_path):
