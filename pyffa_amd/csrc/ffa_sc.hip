@@ -228,11 +228,16 @@ __device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, 
 // SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
 constexpr int MAX_NS_LDS = 4096;
 
+// Particles per pass of k_sc_locate_moments: two keeps it under 128 VGPRs,
+// so all NB blocks are resident at once (the visiting order i0, i0 + S, ...
+// and hence the sums are the same for any value).
+constexpr int LUNR = 2;
+
 // Per survivor: x = r - r0(Ek, phi), pr0 and the slice index (written for
 // the bin and gather kernels), the moments (sum x, z, x^2, z^2) and the
 // slice histogram: per-block counts in LDS, one global atomic per non-empty
 // (block, slice) pair (ns <= FFA_SC_MAX_NS).  Fixed block partition and
-// per-thread order, so the sums are reproducible run to run; UNR particles
+// per-thread order, so the sums are reproducible run to run; LUNR particles
 // per pass have their loads (state, then the SEO table corners) in flight
 // together.
 __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ffa_sc_params sp, double c,
@@ -246,22 +251,22 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
     const double fs = res->fi_s, step = res->slice_step;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t S = (int64_t)NB * BLOCK;
-    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
-        // phases over the UNR particles (flags; state; SEO corners; results),
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += LUNR * S) {
+        // phases over the LUNR particles (flags; state; SEO corners; results),
         // all loads ahead of the stores so the particles' memory round trips
         // overlap
-        bool ok[UNR];
-        int64_t jj[UNR];
+        bool ok[LUNR];
+        int64_t jj[LUNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             const int64_t i = i0 + u * S;
             const uint8_t f = p.flags[i < p.n ? i : 0];
             ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
             jj[u] = ok[u] ? i : 0;
         }
-        double ek[UNR], fm[UNR], x[UNR], z[UNR], fd[UNR], vr[UNR], vz[UNR];
+        double ek[LUNR], fm[LUNR], x[LUNR], z[LUNR], fd[LUNR], vr[LUNR], vz[LUNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             x[u] = p.r[jj[u]];
             fd[u] = p.fidot[jj[u]];
             vr[u] = p.vr[jj[u]];
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             z[u] = p.z[jj[u]];
         }
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             // gamma_of's arithmetic on the loaded state
             double g;
             if (sp.rk) {
@@ -283,15 +288,15 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             ek[u] = (g - 1) * sp.E0_MeV;
             fm[u] = mod2pi(fm[u]);
         }
-        double pr0[UNR];
+        double pr0[LUNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             x[u] -= seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
             pr0[u] = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
         }
-        int s[UNR];
+        int s[LUNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             double f = fm[u];
             if (f < fs) f += FFA_TWO_PI;
             double fsl = (f - fs) / step;
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             }
         }
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
+        for (int u = 0; u < LUNR; ++u) {
             if (s[u] >= 0) { v[0] += x[u]; v[1] += z[u]; v[2] += x[u] * x[u]; v[3] += z[u] * z[u]; }
             wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
         }
