@@ -123,9 +123,10 @@ typedef struct {
     int32_t cell_counts;   /* 1: also store the particles per CIC base cell in the
                               cell_count region (a checking aid; 0 leaves it unwritten) */
     int32_t field_cubes;   /* 1: store the (Ex, Ez) cubes in the exz region and gather
-                              the kicks from them; 0: the gather differentiates phi at
-                              the particle's four cells itself (same arithmetic, same
-                              kicks) and the exz region is left unwritten */
+                              the kicks from them; 0: the library chooses, and on grids
+                              of up to 2^23 cells the gather differentiates phi at the
+                              particle's four cells itself (same arithmetic, same kicks)
+                              and leaves the exz region unwritten */
     /* SEO tables, device, row-major (n_ek, n_fi) */
     const double *seo_r, *seo_pr;
     const double *seo_ek_axis, *seo_perimeter;   /* n_ek each */
