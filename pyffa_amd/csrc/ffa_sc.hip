@@ -105,20 +105,38 @@ __global__ __launch_bounds__(BLOCK) void k_sc_stage1(ffa_particles p, int rk, do
     // still visits its particles i, i + S, i + 2S, ... in that order
     const int64_t S = (int64_t)NB * BLOCK;
     for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        // flags, then every state load, then the arithmetic: no branch between
+        // the particles' loads, so their round trips overlap
         bool ok[UNR];
-        double g[UNR], fi[UNR];
+        int64_t jj[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const int64_t i = i0 + u * S;
-            ok[u] = i < p.n && (p.flags[i < p.n ? i : 0] & FFA_F_ALIVE);
-            const int64_t j = ok[u] ? i : 0;
-            g[u] = ok[u] ? gamma_of(p, j, rk, c, E0J) : 0.0;
-            fi[u] = ok[u] ? p.fi[j] : 0.0;
+            const uint8_t f = p.flags[i < p.n ? i : 0];
+            ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
+            jj[u] = ok[u] ? i : 0;
+        }
+        double r[UNR], fd[UNR], vr[UNR], vz[UNR], fi[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            r[u] = p.r[jj[u]];
+            fd[u] = p.fidot[jj[u]];
+            vr[u] = p.vr[jj[u]];
+            vz[u] = p.vz[jj[u]];
+            fi[u] = p.fi[jj[u]];
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             if (!ok[u]) continue;
-            v[0] += g[u];
+            double g;                                  // gamma_of on the loaded state
+            if (rk) {
+                g = fd[u] / E0J;
+            } else {
+                const double rd = r[u] * fd[u];
+                const double v2 = vr[u] * vr[u] + vz[u] * vz[u] + rd * rd;
+                g = 1.0 / sqrt(1.0 - v2 / (c * c));
+            }
+            v[0] += g;
             v[1] += 1.0;
             const double fm = mod2pi(fi[u]);
             for (int k = 0; k < 4; ++k) {
