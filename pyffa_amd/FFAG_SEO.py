@@ -318,6 +318,7 @@ class FFAG_BetaFuncCalc:
         X[3::5, 2] += d
         X[4::5, 3] += d
         tp, tr = tracker.track(X, np.repeat(ek, 5), 0.0, 2 * sector, h, record=True)
+        self._last_track = (tracker, tp, tr)
         fi0 = np.linspace(0, sector, beta_n)
 
         def at(t):
@@ -341,6 +342,29 @@ class FFAG_BetaFuncCalc:
                     out[2 + p, i, k] = (m11 - m22) / sn / 2
         return fi0, out[0], out[1], out[2], out[3]
 
+    def trajectory_fields(self):
+        """Fields along the last ``twiss`` track, as the reference records
+        them (src/FFAG_BetaFunc.py:82-84, 178-180): Bz of each energy's
+        closed-orbit particle and Br, Bfi of its companion displaced in z,
+        at every recorded azimuth.  Returns (azimuth [rad], Bz, Br, Bfi) with
+        arrays (n_energies, points)."""
+        tracker, tp, tr = self._last_track
+        t = tracker.torch
+        n = tr.shape[1] // 5
+        pts = np.concatenate([tr[:, 0::5, :], tr[:, 3::5, :]], axis=1)          # (points, 2n, 4)
+        m = pts.shape[0] * pts.shape[1]
+        dev = tracker.dev
+        r = t.from_numpy(np.ascontiguousarray(pts[:, :, 0].T.ravel())).to(dev)
+        z = t.from_numpy(np.ascontiguousarray(pts[:, :, 2].T.ravel())).to(dev)
+        fi = t.from_numpy(np.ascontiguousarray(np.tile(np.asarray(tp, dtype=np.float64), 2 * n))).to(dev)
+        out = t.empty(3 * m, dtype=t.float64, device=dev)
+        lib = tracker.ctx.lib
+        tracker.ctx.check(lib.ffa_bfield_eval(tracker.ctx.h, _lib._vp(r.data_ptr()), _lib._vp(fi.data_ptr()),
+                                              _lib._vp(z.data_ptr()), m, _lib._vp(out.data_ptr()), 0, 0.0),
+                          "ffa_bfield_eval")
+        B = out.cpu().numpy().reshape(3, 2 * n, -1)                              # planes Bz, Bfi, Br
+        return np.asarray(tp, dtype=np.float64), B[0, :n], B[2, n:], B[1, n:]
+
     def CalcBetaFunc(self, SEOPATHName, Ek_arr, GlobalParas):
         """Writes BetaFuncR/Z.txt and AlphaFuncR/Z.txt for the energies
         Ek_arr at the closed orbits of SEO_ini.txt (linear in Ek)."""
@@ -357,5 +381,13 @@ class FFAG_BetaFuncCalc:
             m = np.column_stack((head, np.vstack((np.rad2deg(fi0), tab))))
             with open(os.path.join(SEOPATHName, name), "w") as f:
                 f.write("unitEk=MeV, unitFi=deg\n")
+                np.savetxt(f, m)
+        # the trajectory fields; like the reference, the first row holds the
+        # tracking azimuth as tracked (radians) under a "unitFi=deg" header
+        tp, Bz, Br, Bf = self.trajectory_fields()
+        for name, tab in (("Bz.txt", Bz), ("Br.txt", Br), ("Bf.txt", Bf)):
+            m = np.column_stack((head, np.vstack((tp, tab))))
+            with open(os.path.join(SEOPATHName, name), "w") as f:
+                f.write("unitEk=MeV, unitFi=deg, unitB=T\n")
                 np.savetxt(f, m)
         return fi0, br, bz, ar, az

@@ -61,13 +61,26 @@ def test_closed_orbits_tunes_and_twiss():
     for i, e in enumerate(eks):
         oq = OB.tunes(mc, r0[i], pr0[i], e)
         np.testing.assert_allclose([qr[i], qz[i], f[i], mr[i], per[i]], oq, rtol=1e-9)
-    fi0, br, bz, ar, az = FFAG_BetaFuncCalc().twiss(gp, r0, pr0, eks, beta_n=7)
+    bf = FFAG_BetaFuncCalc()
+    fi0, br, bz, ar, az = bf.twiss(gp, r0, pr0, eks, beta_n=7)
     for i, e in enumerate(eks):
         ot = OB.twiss(mc, r0[i], pr0[i], e, fi0)
         np.testing.assert_allclose(br[i], ot[:, 0], rtol=1e-6)
         np.testing.assert_allclose(bz[i], ot[:, 1], rtol=1e-6)
         np.testing.assert_allclose(ar[i], ot[:, 2], rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(az[i], ot[:, 3], rtol=1e-5, atol=1e-6)
+    # fields along the two-cell track: Bz of the closed-orbit particle, Br and
+    # Bfi of the z-displaced one, against the oracle's field at the same points
+    tp, tBz, tBr, tBf = bf.trajectory_fields()
+    _, _, tr = bf._last_track
+    assert tBz.shape == (eks.size, tp.size) and tp[0] == 0.0 and abs(tp[-1] - np.pi / 3) < 1e-12
+    for i in range(eks.size):
+        oz, _, _ = OB.fields(mc, tr[:, 5 * i, 0], tp, tr[:, 5 * i, 2])
+        _, orr, of = OB.fields(mc, tr[:, 5 * i + 3, 0], tp, tr[:, 5 * i + 3, 2])
+        np.testing.assert_allclose(tBz[i], oz, rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(tBr[i], orr, rtol=1e-12, atol=1e-22)
+        np.testing.assert_allclose(tBf[i], of, rtol=1e-12, atol=1e-22)
+        assert np.max(np.abs(tBr[i])) > 0 and np.max(np.abs(tBf[i])) > 0     # z = 1e-7 m: small but not zero
 
 
 def test_uniform_field_closed_forms_on_the_gpu():
@@ -117,3 +130,10 @@ def test_seo_files_feed_the_tracker_and_the_bunch_generator(tmp_path):
     FFAG_BetaFuncCalc().CalcBetaFunc(folder, eks, gp)
     bz_, br_, az_, ar_ = FD.LoadSEOBetaFunc(folder, 300.0, 0.0)
     assert 0 < br_ < 20 and 0 < bz_ < 30 and abs(ar_) < 5 and abs(az_) < 5
+    for name in ("Bz.txt", "Br.txt", "Bf.txt"):
+        with open(os.path.join(folder, name)) as f:
+            assert f.readline().startswith("unitEk=MeV, unitFi=deg, unitB=T")
+        tab = np.loadtxt(os.path.join(folder, name), skiprows=1)
+        assert tab.shape[0] == eks.size + 1 and np.array_equal(tab[1:, 0], eks) and tab[0, 1] == 0.0
+    bzt = np.loadtxt(os.path.join(folder, "Bz.txt"), skiprows=1)
+    assert np.all(np.abs(bzt[1:, 1:]) < 5.0) and np.ptp(bzt[1:, 1:]) > 0.1     # the F/D pattern along the orbit
