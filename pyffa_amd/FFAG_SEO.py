@@ -49,9 +49,10 @@ def _grid(t0, t1, h):
 
 class OrbitTracker:
     """GPU tracking of (r, pr, z, pz) in azimuth for the field of
-    ``GlobalParas`` (order 0 + harmonics)."""
+    ``GlobalParas`` (order 0, plus the harmonic errors unless
+    ``harmonics`` is False)."""
 
-    def __init__(self, GlobalParas, device=None):
+    def __init__(self, GlobalParas, device=None, harmonics=True):
         import torch
         self.torch = torch
         if not torch.cuda.is_available():
@@ -59,7 +60,8 @@ class OrbitTracker:
         self.gp = GlobalParas
         self.dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self.ctx = _lib.Context(self.dev.index, torch.cuda.current_stream(self.dev).cuda_stream)
-        light = types.SimpleNamespace(Bmap=GlobalParas.Bmap, BHarmonics=GlobalParas.BHarmonics, Emap=None,
+        light = types.SimpleNamespace(Bmap=GlobalParas.Bmap,
+                                      BHarmonics=GlobalParas.BHarmonics if harmonics else None, Emap=None,
                                       Aperture_enable=False, q=GlobalParas.q, c=GlobalParas.c, m0=GlobalParas.m0,
                                       E0=GlobalParas.E0)
         m = build_machine(light, None, False)
@@ -306,7 +308,9 @@ class FFAG_BetaFuncCalc:
     def twiss(self, GlobalParas, r0, pr0, ek, beta_n=1000, h=0.001):
         """(fi0 [rad], beta_r, beta_z, alpha_r, alpha_z) with arrays
         (n_energies, beta_n) over one sector."""
-        tracker = OrbitTracker(GlobalParas)
+        # the reference tracks the Twiss cells in a parameter set holding the
+        # field map alone (src/FFAG_BetaFunc.py:76-80): no harmonic errors
+        tracker = OrbitTracker(GlobalParas, harmonics=False)
         r0, pr0, ek = (np.atleast_1d(np.asarray(a, dtype=np.float64)) for a in (r0, pr0, ek))
         n = ek.size
         sector = np.pi * 2 / GlobalParas.Bmap.Nsectors

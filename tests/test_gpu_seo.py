@@ -69,6 +69,9 @@ def test_closed_orbits_tunes_and_twiss():
         np.testing.assert_allclose(bz[i], ot[:, 1], rtol=1e-6)
         np.testing.assert_allclose(ar[i], ot[:, 2], rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(az[i], ot[:, 3], rtol=1e-5, atol=1e-6)
+    # as in the reference, the Twiss cells are tracked in the field map alone
+    _, gph = _gp([dict(m=3, amp_Gs=2.0, phase_deg=30.0, rmin=8.0, rmax=9.3)])
+    assert np.array_equal(FFAG_BetaFuncCalc().twiss(gph, r0, pr0, eks, beta_n=7)[1], br)
     # fields along the two-cell track: Bz of the closed-orbit particle, Br and
     # Bfi of the z-displaced one, against the oracle's field at the same points
     tp, tBz, tBr, tBf = bf.trajectory_fields()
