@@ -122,6 +122,9 @@ typedef struct {
     int32_t bits;          /* fixed-point fractional bits per CIC axis, 1..20 */
     int32_t cell_counts;   /* 1: also store the particles per CIC base cell in the
                               cell_count region (a checking aid; 0 leaves it unwritten) */
+    int32_t charge_grid;   /* 1: also store the int64 fixed-point charge grid in the grid
+                              region (the deposit always writes the scaled float32
+                              charge straight into the packed FFT buffer) */
     int32_t field_cubes;   /* 1: store the (Ex, Ez) cubes in the exz region and gather
                               the kicks from them; 0: the library chooses, and on grids
                               of up to 2^23 cells the gather differentiates phi at the
@@ -151,7 +154,7 @@ typedef struct {
  * byte offset of each region inside one caller-allocated block. */
 typedef struct {
     size_t total;
-    size_t grid;        /* int64 (ns, nz, nx) fixed-point charge */
+    size_t grid;        /* int64 (ns, nz, nx) fixed-point charge (when charge_grid asks) */
     size_t cell_count;  /* int32 (ns, nz, nx) particles per CIC base cell */
     size_t cbuf;        /* complex64 (ceil(ns/2), nz, nx) packed slice pairs */
     size_t phi;         /* float (ns, nz, nx) potential */

@@ -109,6 +109,7 @@ class FFAG_Bunch:
         self._host_cache = None
         # store per-cell CIC base counts at each solve (a checking aid)
         self.sc_cell_counts = False
+        self.sc_charge_grid = False     # store the int64 fixed-point charge grid of each solve
         self.sc_field_cubes = False     # store the (Ex, Ez) cubes of each solve
 
     # ------------------------------------------------------------------ state
@@ -428,6 +429,7 @@ class FFAG_Bunch:
         sp.s_last = float(s_last)
         sp.bits = fixed_point_bits(max(self.TotalParticleNum_Local, 1))
         sp.cell_counts = 1 if self.sc_cell_counts else 0
+        sp.charge_grid = 1 if self.sc_charge_grid else 0
         sp.field_cubes = 1 if self.sc_field_cubes else 0
         sp.seo_r, sp.seo_pr = seo["r"].data_ptr(), seo["pr"].data_ptr()
         sp.seo_ek_axis, sp.seo_perimeter = seo["ek"].data_ptr(), seo["per"].data_ptr()
@@ -443,7 +445,8 @@ class FFAG_Bunch:
     def sc_view(self, name):
         """Typed device view of one workspace region (after a solve); "ex"
         and "ez" are the components of the interleaved field region, written
-        only by solves with ``sc_field_cubes`` set."""
+        only by solves with ``sc_field_cubes`` set; "grid" is written only by
+        solves with ``sc_charge_grid`` set."""
         t = self.torch
         if name in ("ex", "ez"):
             return self.sc_view("exz")[..., 0 if name == "ex" else 1]
@@ -476,12 +479,12 @@ class FFAG_Bunch:
         unused.  Returns the reference's 18-tuple with host (numpy) arrays;
         the tracking loop calls ``sc_solve_device`` instead, which copies
         nothing back."""
-        keep = self.sc_field_cubes
-        self.sc_field_cubes = True
+        keep = self.sc_field_cubes, self.sc_charge_grid
+        self.sc_field_cubes = self.sc_charge_grid = True
         try:
             self.sc_solve_device(ctx, first=first, s_last=s_last)
         finally:
-            self.sc_field_cubes = keep
+            self.sc_field_cubes, self.sc_charge_grid = keep
         res = self.sc_result()
         rho = self.sc_view("grid").cpu().numpy().astype(np.float64) * (
             res["qscale"] / float(1 << (2 * fixed_point_bits(max(self.TotalParticleNum_Local, 1)))))

@@ -65,7 +65,7 @@ class FfaStep(ctypes.Structure):
 
 class FfaScParams(ctypes.Structure):
     _fields_ = [("ns", _i32), ("nx", _i32), ("nz", _i32), ("sigma_ratio", _d), ("q_macro", _d),
-                ("rk", _i32), ("first", _i32), ("s_last", _d), ("bits", _i32), ("cell_counts", _i32), ("field_cubes", _i32),
+                ("rk", _i32), ("first", _i32), ("s_last", _d), ("bits", _i32), ("cell_counts", _i32), ("charge_grid", _i32), ("field_cubes", _i32),
                 ("seo_r", _vp), ("seo_pr", _vp), ("seo_ek_axis", _vp), ("seo_perimeter", _vp),
                 ("n_ek", _i32), ("n_fi", _i32), ("ek0", _d), ("dek", _d), ("fi0", _d), ("dfi", _d),
                 ("E0_MeV", _d), ("c", _d)]

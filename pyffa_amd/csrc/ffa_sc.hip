@@ -15,11 +15,12 @@
 //   k_sc_rowsort       per slice: records ordered by CIC row, row offsets;
 //                      the blocks also fill the per-solve FFT twiddles and
 //                      Green's function table
-//   k_sc_deposit_tiles one workgroup per (slice, z band): LDS int64 / int32
-//                      accumulation of the slice's records, plain stores of
-//                      the band (integer sums: bit-identical for any order,
-//                      and no device-scope atomics on the grid)
-//   k_fft_rows_fwd     int64 -> float, two slices packed as re/im, row FFTs
+//   k_sc_deposit_tiles one workgroup per (slice pair, z band): LDS int64
+//                      accumulation of the two slices' records (integer sums:
+//                      bit-identical for any order, no device-scope atomics),
+//                      then plain stores of the scaled float32 charge packed
+//                      as re/im into the FFT buffer
+//   k_fft_rows_fwd     row FFTs of the packed slice pairs, in place
 //   k_fft_cols_green   column FFT, x periodic Green's function, inverse
 //   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi
 //   k_sc_field         -grad(phi)/gamma^2 with the reference's edge stencils,
@@ -537,53 +538,73 @@ __global__ __launch_bounds__(ROW_BLOCK) void k_sc_rowsort(ffa_sc_params sp, cons
         }
 }
 
-// One workgroup per (slice, band of bz rows): zero an LDS tile, add the
-// corners of the records of rows z0 - 1 .. z1 - 1 that fall in the band (a
-// contiguous range after k_sc_rowsort), store the band.  Dynamic LDS:
-// bz * nx int64 sums, then (when sp.cell_counts) bz * nx int32 base-cell
-// counts.
+// One workgroup per (slice pair, band of bz rows): zero an LDS tile per
+// slice, add the corners of the records of rows z0 - 1 .. z1 - 1 that fall
+// in the band (a contiguous range of each slice after k_sc_rowsort), then
+// store the band as the packed FFT input: charge x scale in float32, the pair
+// as (re, im), in the rounding the row FFT applied when it read the integer
+// grid.  The int64 grid (sp.charge_grid) and the per-cell counts
+// (sp.cell_counts) are stored only on request.  Dynamic LDS: 2 bz nx int64
+// sums, then (with counts) 2 bz nx int32 base-cell counts.
 __global__ __launch_bounds__(BLOCK) void k_sc_deposit_tiles(ffa_sc_params sp, const ffa_sc_result *res,
                                                            const int32_t *row_start, const uint64_t *rec,
-                                                           unsigned long long *grid, int32_t *cell, int bz) {
+                                                           unsigned long long *grid, int32_t *cell, float2 *cbuf,
+                                                           int bz) {
     extern __shared__ unsigned long long lds[];
     const int nx = sp.nx, nz = sp.nz;
     const int nbands = nz / bz;
-    const int s = blockIdx.x / nbands, band = blockIdx.x - s * nbands;
+    const int pl = blockIdx.x / nbands, band = blockIdx.x - pl * nbands;
     const int z0 = band * bz, z1 = z0 + bz;
     const int ncell = bz * nx;
     const bool counts = sp.cell_counts != 0;
+    const int npair = 2 * pl + 1 < sp.ns ? 2 : 1;          // an odd ns leaves the last plane one slice
     unsigned long long *acc = lds;
-    int32_t *cnt = reinterpret_cast<int32_t *>(lds + ncell);
-    for (int k = threadIdx.x; k < ncell; k += BLOCK) acc[k] = 0ull;
+    int32_t *cnt = reinterpret_cast<int32_t *>(lds + 2 * ncell);
+    for (int k = threadIdx.x; k < 2 * ncell; k += BLOCK) acc[k] = 0ull;
     if (counts)
-        for (int k = threadIdx.x; k < ncell; k += BLOCK) cnt[k] = 0;
+        for (int k = threadIdx.x; k < 2 * ncell; k += BLOCK) cnt[k] = 0;
     __syncthreads();
     if (res->bad_grid == 0.0) {
         const uint64_t one = (uint64_t)1 << sp.bits;
-        const int32_t *rs = row_start + (size_t)s * (nz + 1);
-        const int r0 = rs[z0 > 0 ? z0 - 1 : 0], r1 = rs[z1];
-        for (int t = r0 + threadIdx.x; t < r1; t += BLOCK) {
-            const uint64_t w = rec[t];
-            const int ix = (int)(w & 1023u), iz = (int)((w >> 10) & 1023u);
-            const uint64_t a = (w >> 20) & 0x1FFFFFu, b = (w >> 41) & 0x1FFFFFu;
-            if (iz >= z0 && iz < z1) {
-                const int o = (iz - z0) * nx + ix;
-                atomicAdd(&acc[o], (one - a) * (one - b));
-                atomicAdd(&acc[o + 1], a * (one - b));
-                if (counts) atomicAdd(&cnt[o], 1);
-            }
-            if (iz + 1 >= z0 && iz + 1 < z1) {
-                const int o = (iz + 1 - z0) * nx + ix;
-                atomicAdd(&acc[o], (one - a) * b);
-                atomicAdd(&acc[o + 1], a * b);
+        for (int t = 0; t < npair; ++t) {
+            const int s = 2 * pl + t;
+            unsigned long long *a_t = acc + t * ncell;
+            int32_t *c_t = cnt + t * ncell;
+            const int32_t *rs = row_start + (size_t)s * (nz + 1);
+            const int r0 = rs[z0 > 0 ? z0 - 1 : 0], r1 = rs[z1];
+            for (int q = r0 + threadIdx.x; q < r1; q += BLOCK) {
+                const uint64_t w = rec[q];
+                const int ix = (int)(w & 1023u), iz = (int)((w >> 10) & 1023u);
+                const uint64_t a = (w >> 20) & 0x1FFFFFu, b = (w >> 41) & 0x1FFFFFu;
+                if (iz >= z0 && iz < z1) {
+                    const int o = (iz - z0) * nx + ix;
+                    atomicAdd(&a_t[o], (one - a) * (one - b));
+                    atomicAdd(&a_t[o + 1], a * (one - b));
+                    if (counts) atomicAdd(&c_t[o], 1);
+                }
+                if (iz + 1 >= z0 && iz + 1 < z1) {
+                    const int o = (iz + 1 - z0) * nx + ix;
+                    atomicAdd(&a_t[o], (one - a) * b);
+                    atomicAdd(&a_t[o + 1], a * b);
+                }
             }
         }
     }
     __syncthreads();
-    const size_t base = (size_t)s * nz * nx + (size_t)z0 * nx;
-    for (int k = threadIdx.x; k < ncell; k += BLOCK) grid[base + k] = acc[k];
-    if (counts)
-        for (int k = threadIdx.x; k < ncell; k += BLOCK) cell[base + k] = cnt[k];
+    const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
+    float2 *cb = cbuf + (size_t)pl * nz * nx + (size_t)z0 * nx;
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) {
+        const float re = (float)((double)(int64_t)acc[k] * us);
+        const float im = npair == 2 ? (float)((double)(int64_t)acc[ncell + k] * us) : 0.0f;
+        cb[k] = make_float2(re, im);
+    }
+    for (int t = 0; t < npair; ++t) {
+        const size_t base = (size_t)(2 * pl + t) * nz * nx + (size_t)z0 * nx;
+        if (sp.charge_grid)
+            for (int k = threadIdx.x; k < ncell; k += BLOCK) grid[base + k] = acc[t * ncell + k];
+        if (counts)
+            for (int k = threadIdx.x; k < ncell; k += BLOCK) cell[base + k] = cnt[t * ncell + k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -651,26 +672,16 @@ __device__ float2 *lds_fft4(float2 *x, float2 *y, const float2 *tw, int N, int l
     return x;
 }
 
-// rows: forward FFT along x of the packed slice pair; grid -> cbuf.
-__global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(const int64_t *grid, float2 *cbuf, ffa_sc_params sp,
-                                                       const ffa_sc_result *res, const float2 *twt, int lognx,
-                                                       int lognz) {
+// rows: forward FFT along x of the packed slice pairs, in place in cbuf.
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(float2 *cbuf, ffa_sc_params sp, const float2 *twt,
+                                                       int lognx) {
     extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
     float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
-    const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
+    const int nx = sp.nx;
     const int lines = FFT_PTS / nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
-    const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
     load_twiddles(tw, twt, nx);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
-        const int l = e >> lognx, i = e & (nx - 1);
-        const int64_t row = row0 + l;
-        const int64_t plane = row >> lognz, j = row & (nz - 1);
-        const int64_t sa = 2 * plane, sb = sa + 1;
-        const float re = (float)((double)grid[(sa * nz + j) * nx + i] * us);
-        const float im = sb < ns ? (float)((double)grid[(sb * nz + j) * nx + i] * us) : 0.0f;
-        b0[e] = make_float2(re, im);
-    }
+    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
     __syncthreads();
     const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, false);
     for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) cbuf[row0 * nx + e] = o[e];
@@ -980,23 +991,23 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
                        slc, blk_off, part);
     hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart);
     hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, sstart, blk_off, rec);
-    const int bz = (4096 / sp->nx) < sp->nz ? (4096 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
-    const size_t tile_lds = (size_t)bz * sp->nx * (sp->cell_counts ? 12 : 8);
+    const int bz = (2048 / sp->nx) < sp->nz ? (2048 / sp->nx) : sp->nz;     // <= 48 KB of LDS per tile
+    const size_t tile_lds = (size_t)2 * bz * sp->nx * (sp->cell_counts ? 12 : 8);
     uint64_t *rec2 = reinterpret_cast<uint64_t *>(w + L->records_by_row);
     int32_t *rstart = reinterpret_cast<int32_t *>(w + L->row_start);
     float2 *twt = reinterpret_cast<float2 *>(w + L->tables);
     float *green = reinterpret_cast<float *>(w + L->tables + 8 * 4 * (size_t)TW_STRIDE);
     hipLaunchKernelGGL(k_sc_rowsort, dim3((unsigned)sp->ns), dim3(ROW_BLOCK), 0, st, *sp, res, sstart, rec, rec2,
                        rstart, twt, green);
-    hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st, *sp,
-                       res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
+    hipLaunchKernelGGL(k_sc_deposit_tiles, dim3((unsigned)(((sp->ns + 1) / 2) * (sp->nz / bz))), dim3(BLOCK), tile_lds,
+                       st, *sp, res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, cbuf, bz);
     const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
     const int64_t planes = (sp->ns + 1) / 2;
     const int64_t rows = planes * sp->nz;
     const size_t rows_lds = sizeof(float2) * (2 * FFT_PTS + sp->nx);
     const size_t cols_lds = sizeof(float2) * (2 * (COL_PTS + 32) + 2 * sp->nz);
-    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, grid, cbuf,
-                       *sp, res, twt, lognx, lognz);
+    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf, *sp,
+                       twt, lognx);
     hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK),
                        cols_lds, st, cbuf, *sp, twt, green, lognz);
     hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf, phi,

@@ -131,6 +131,7 @@ def test_sc_solve_parity(torch_cuda, grid):
     B = FFAG_Bunch(M, marcosize=marcosize, sc_grid_size=(ns, nx, nz, 4.0), BunchType="Boris")
     cases.set_bunch_seo(B)
     B.sc_cell_counts = True
+    B.sc_charge_grid = True
     B.sc_field_cubes = True
     B.sc_solve_device(first=True, s_last=0.0)
     res = B.sc_result()
