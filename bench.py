@@ -487,7 +487,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=50)
-    ap.add_argument("--train-iters", type=int, default=6000)
+    ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--scale-particles", type=int, default=8_000_000)
     ap.add_argument("--scale-steps", type=int, default=10)

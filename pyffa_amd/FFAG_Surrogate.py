@@ -23,7 +23,7 @@ Typical use::
 
     sur = FFAG_FieldSurrogate(gp_order4, zt=0.01)   # gp_order4.Bmap has max_order 4
     sur.build_dataset(4_000_000)
-    sur.train(iters=6000)
+    sur.train(iters=12000)
     print(sur.field_error())
     LocalParameters["surrogate"] = sur               # FP32 tier uses it
 """
@@ -198,7 +198,7 @@ class FFAG_FieldSurrogate:
                                               _lib._vp(self.m.data_ptr()), _lib._vp(self.v.data_ptr()),
                                               _lib._vp(self.packed.data_ptr()), ctypes.byref(hp)), "ffa_mlp_adam")
 
-    def train(self, iters=6000, batch=8192, lr=2e-3, lr_final=0.0, seed=2, log_every=500, verbose=False):
+    def train(self, iters=12000, batch=8192, lr=2e-3, lr_final=0.0, seed=2, log_every=500, verbose=False):
         """Adam with a cosine learning-rate schedule over ``iters`` steps on
         reshuffled passes over the dataset; returns the loss history
         [(iteration, loss)]."""
