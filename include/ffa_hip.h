@@ -122,8 +122,9 @@ typedef struct {
     int32_t bits;          /* fixed-point fractional bits per CIC axis, 1..20 */
     int32_t cell_counts;   /* 1: also store the particles per CIC base cell in the
                               cell_count region (a checking aid; 0 leaves it unwritten) */
-    int32_t charge_grid;   /* 1: also store the int64 fixed-point charge grid in the grid
-                              region (the deposit always writes the scaled float32
+    int32_t charge_grid;   /* 1: store the int64 fixed-point charge grid in the grid
+                              region; 0: the library chooses (grids of up to 2^23
+                              cells store it, larger ones deposit the scaled float32
                               charge straight into the packed FFT buffer) */
     int32_t field_cubes;   /* 1: store the (Ex, Ez) cubes in the exz region and gather
                               the kicks from them; 0: the library chooses, and on grids

@@ -115,10 +115,12 @@ def test_rk4_f64_parity(torch_cuda, tmp_path):
         assert max(err.values()) < tol, (nsteps, err)
 
 
-@pytest.mark.parametrize("grid", [(32, 64, 64), (33, 32, 128)])
+@pytest.mark.parametrize("grid", [(32, 64, 64), (33, 32, 128), (129, 256, 256)])
 def test_sc_solve_parity(torch_cuda, grid):
     """One SC solve: scalars, bit-exact fixed-point grid and per-cell counts,
-    potential/fields and particle kicks."""
+    potential/fields and particle kicks.  The last grid is above 2^23 cells:
+    the deposit writes slice pairs into the FFT buffer and the gather reads
+    the field cubes."""
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     ns, nz, nx = grid
     print(f"[case] sc solve grid={grid}", flush=True)
