@@ -352,6 +352,8 @@ class FFAG_BetaFuncCalc:
         closed-orbit particle and Br, Bfi of its companion displaced in z,
         at every recorded azimuth.  Returns (azimuth [rad], Bz, Br, Bfi) with
         arrays (n_energies, points)."""
+        if getattr(self, "_last_track", None) is None:
+            raise RuntimeError("trajectory_fields() needs a twiss() track first")
         tracker, tp, tr = self._last_track
         t = tracker.torch
         n = tr.shape[1] // 5
