@@ -396,7 +396,7 @@ def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
     return out
 
 
-def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p")):
+def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False):
     """Incoherent tunes with space charge: the bunch of ``accuracy_leg`` is
     tracked ``turns`` + 1.5 turns in each run ("bare": FP64 without SC;
     "fp64", "fp32": with SC; "fp64p": FP64 with SC from r and z moved by one
@@ -433,9 +433,10 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p")):
         dumps.add_dump(pd)
         rk = FFAG_RungeKutta()
         t0 = time.perf_counter()
+        log(f"tune {run}: tracking {n} particles for {total} steps")
         lp = dict(stop_condition=dict(max_stepsN=total, max_turn=1e9, max_time=1.0), time_step=DT,
                   step_dumps=dumps, enable_SC=run != "bare", SC_type=1, SC_step=SC_STEP, precision=prec,
-                  verbose=False)
+                  verbose=progress)
         rk.rk4_solve_dt_bunch3_withSC_boris(None, 0.0, B, gp, lp)
         pd.save_all_particles()
         ids, qx, qz = D.particle_tunes(D.load_position_dump(folder, 0.0), seo, turns=turns)
@@ -503,6 +504,8 @@ def main():
     ap.add_argument("--tune-particles", type=int, default=100_000)
     ap.add_argument("--tune-tracked", type=int, default=256)
     ap.add_argument("--only-tune", action="store_true", help="run the tune leg alone")
+    ap.add_argument("--tune-progress", action="store_true",
+                    help="the tune leg's tracking prints its progress lines (on stdout, before the JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -522,7 +525,8 @@ def main():
         print(json.dumps({"map_generator": mapgen_leg(cpu=not args.no_cpu)}), flush=True)
         return
     if args.only_tune:
-        print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked)}),
+        print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
+                                                 progress=args.tune_progress)}),
               flush=True)
         return
     if args.only_accuracy:
@@ -582,7 +586,8 @@ def main():
     if not args.no_mapgen:
         out["map_generator"] = mapgen_leg(cpu=not args.no_cpu)
     if args.tune_turns > 0:
-        out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked)
+        out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
+                                     progress=args.tune_progress)
     if args.accuracy_turns > 0:
         out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
                                             args.accuracy_tiers.split(","))
