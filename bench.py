@@ -281,6 +281,11 @@ def orbit_moments(S, seo):
                 alive=int(al.sum()))
 
 
+def _grid_arg(text):
+    """'ns,nx,nz' -> (ns, nx, nz, sigma_ratio) like GRID, or GRID itself."""
+    return GRID if not text else tuple(int(v) for v in text.split(",")) + (GRID[3],)
+
+
 def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
     """Tracks the same bunch for ``turns`` turns (SC, RF, aperture) in the
     requested runs: "fp32" (FP32 tier), "fp64" (FP64 path) and "fp64p" (FP64
@@ -496,6 +501,8 @@ def main():
                     help="also run both tiers this many turns and report the moments' deviation (off by default)")
     ap.add_argument("--accuracy-particles", type=int, default=100_000)
     ap.add_argument("--accuracy-tiers", default="fp32,fp64,fp64p")
+    ap.add_argument("--accuracy-grid", default=None, help="SC grid ns,nx,nz of the accuracy leg")
+    ap.add_argument("--accuracy-chunk", type=float, default=50.0, help="turns between moment samples")
     ap.add_argument("--only-accuracy", action="store_true", help="run the accuracy leg alone")
     ap.add_argument("--no-mapgen", action="store_true", help="skip the Taylor-map generator leg")
     ap.add_argument("--only-mapgen", action="store_true", help="run the Taylor-map generator leg alone")
@@ -530,7 +537,8 @@ def main():
               flush=True)
         return
     if args.only_accuracy:
-        acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","))
+        acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","),
+                           chunk_turns=args.accuracy_chunk, grid=_grid_arg(args.accuracy_grid))
         print(json.dumps({"accuracy_long": acc}), flush=True)
         return
     from tests import cases
@@ -590,7 +598,8 @@ def main():
                                      progress=args.tune_progress)
     if args.accuracy_turns > 0:
         out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
-                                            args.accuracy_tiers.split(","))
+                                            args.accuracy_tiers.split(","), chunk_turns=args.accuracy_chunk,
+                                            grid=_grid_arg(args.accuracy_grid))
     if not args.no_cpu:
         cb = cpu_baseline(M, mc, min(args.cpu_particles, M.shape[0]), args.cpu_steps)
         log(f"cpu baseline {cb}")
