@@ -1,23 +1,31 @@
 """Throughput benchmark of the FFAG tracking hot path on one MI355X.
 
 Workload (headline): 12-sector scaling FFAG (r0 = 8.5 m, k = 7.6, 300 MeV
-protons, order-2 Taylor maps on an r 8.0-9.3 m x one-sector grid), 1M
-macro-particles (seeded Gaussian bunch, uniform azimuth), 2.5D space charge
-on a (ns, nx, nz) = (256, 128, 128) grid at +-4 sigma every 20 push steps,
-two RF gaps 180 degrees apart at 100 kV with a linear frequency ramp, Boris
-pusher, dt = 0.05 ns.  One bench step = 20 push steps + one SC solve.
+protons, order-2 Taylor maps on an r 8.0-9.3 m x one-sector grid), 8M
+macro-particles (seeded Gaussian bunch, uniform azimuth, 1e13 protons), 2.5D
+space charge on a (ns, nx, nz) = (512, 256, 256) grid at +-4 sigma every 20
+push steps, two RF gaps 180 degrees apart at 100 kV with a linear frequency
+ramp, the aperture r 8.0-9.3 m |z| < 5 cm, Boris pusher, dt = 0.05 ns.
 
-The headline value is the FP32-field tier (FP64 state, fused steps); the FP64
-map-gather path is timed on the same workload for the speed-up, and the
-accuracy of the FP32 tier is reported against it (centroid, rms emittance).
+One bench step = 1000 Boris push steps: the reference's progress / stop-check
+interval (src/FFAG_track.py:518-577).  It holds 50 SC solves, one stop-check
+reduction and, in the tiers that order particles, one azimuth sort.
 
-Further legs (each a sub-object of the JSON line):
+The headline value is the FP32-field tier (FP64 state, fused steps).  Beside
+it, on the same workload: the FP64 map-gather path in the reference's
+particle order (the parity path) and the FP64 path with the same azimuth
+ordering as the FP32 tier, so the speed-ups separate what FP32 buys from
+what ordering buys; the FP32 tier's accuracy against FP64 (centroid, rms
+emittance) and its field error against the FP64 field at sampled points.
+
+Further legs (sub-objects of the JSON line):
 * ``surrogate`` -- the field network trained on the GPU on 4M samples of the
-  order-4 Taylor-map field, driving the FP32 tier on the same workload with
-  order-4 maps as fallback; beside it the FP32 and FP64 order-4 map tiers;
-  training time, field error, matrix-core throughput of the network;
-* ``scale_8m`` -- 8M particles with a (512, 256, 256) grid, FP32 vs FP64;
-* ``cpu_baseline`` -- the OpenMP FP64 oracle on the host cores.
+  order-4 Taylor-map field, driving the FP32 tier on the same 8M workload
+  with order-4 maps as fallback; beside it the FP32 and FP64 order-4 map
+  tiers; training time, field error, matrix-core throughput of the network;
+* ``map_generator`` -- Taylor-map generation on the full map grid;
+* ``cpu_baseline`` -- the OpenMP FP64 oracle on the host cores, on a sample
+  of the same workload.
 """
 import argparse
 import json
@@ -28,10 +36,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N_PART = 1_000_000
-GRID = (256, 128, 128, 4.0)      # (ns, nx, nz, sigma_ratio)
-GRID_8M = (512, 256, 256, 4.0)
+N_PART = 8_000_000
+GRID = (512, 256, 256, 4.0)      # (ns, nx, nz, sigma_ratio)
 SC_STEP = 20
+PUSHES = 1000                    # push steps per bench step
 DT = 0.05e-9
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
@@ -39,8 +47,8 @@ BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP = 2 * (64 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_push_1M.json")
-VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_fused_push_1M.json")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_push_8M.json")
+VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_fused_push_8M.json")
 
 
 def measured_traffic(kernel):
@@ -71,8 +79,9 @@ def log(msg):
 
 
 def machines(max_order, tmpdir):
-    from tests import cases
-    return cases.machines("stable", max_order=max_order, rf=dict(v0=100e3), aperture=cases.APERTURE, tmpdir=tmpdir)
+    """FFAG_GlobalParameters of the workload with Taylor maps to max_order."""
+    from pyffa_amd import workloads as W
+    return W.global_parameters("stable", max_order=max_order, rf=dict(v0=100e3), aperture=W.APERTURE, tmpdir=tmpdir)
 
 
 def snapshot(B):
@@ -104,21 +113,27 @@ def accuracy(Sa, Sb, steps):
             "compared": ma["alive"], "after_steps": steps}
 
 
-def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None):
+def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_every=None, marcosize=None):
+    """Tracks the bunch ``warmup`` untimed then ``steps`` timed bench steps
+    (PUSHES push steps each) through the tracking loop; returns the bunch,
+    the tracker, the timed seconds and the final state in input order."""
     import torch
+    from pyffa_amd import workloads as W
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
-    from tests import cases
-    B = FFAG_Bunch(M, marcosize=1e13 / M.shape[0], sc_grid_size=grid, BunchType="Boris", precision=precision)
-    cases.set_bunch_seo(B)
+    B = FFAG_Bunch(M, marcosize=marcosize or 1e13 / M.shape[0], sc_grid_size=grid, BunchType="Boris",
+                   precision=precision)
+    W.set_bunch_seo(B)
     rk = FFAG_RungeKutta()
 
     def lp(first, last):
-        d = dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0),
+        d = dict(stop_condition=dict(max_stepsN=last * PUSHES, max_turn=1e9, max_time=1.0),
                  time_step=DT, step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP,
-                 precision=precision, verbose=False, start_step=first * SC_STEP)
+                 precision=precision, verbose=False, start_step=first * PUSHES)
         if surrogate is not None:
             d["surrogate"] = surrogate
+        if sort_every is not None:
+            d["sort_every"] = sort_every
         return d
     t = 0.0
     if warmup:
@@ -173,28 +188,34 @@ def kernel_time_ms(rk, B, gp, precision, reps=10):
     return res
 
 
-def cpu_baseline(M, mc, n_sample, macro_steps):
+def cpu_baseline(M, gp, n_sample, blocks, n_total):
+    """The FP64 OpenMP oracle on the host cores, on a bounded sample of the
+    workload: its first ``n_sample`` particles (macro-particle charge of the
+    full bunch) for ``blocks`` x SC_STEP Boris steps with one SC solve per
+    SC_STEP steps on the workload's grid."""
     from oracle import oracle as O
     from tests import cases
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     O.lib().ora_set_threads(threads)
     S = M[:n_sample].copy()
-    tr = O.Tracker(mc, S, enable_sc=True, sc_step=SC_STEP, grid=GRID, marcosize=1e13 / n_sample,
+    tr = O.Tracker(cases.oracle_machine(gp), S, enable_sc=True, sc_step=SC_STEP, grid=GRID, marcosize=1e13 / n_total,
                    seo=cases.oracle_seo())
     t0 = time.perf_counter()
-    tr.run(0.0, DT, macro_steps * SC_STEP)
+    tr.run(0.0, DT, blocks * SC_STEP)
     el = time.perf_counter() - t0
-    return dict(value=n_sample * macro_steps * SC_STEP / el, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{n_sample} particles x {macro_steps * SC_STEP} Boris steps with "
-                       f"{macro_steps} SC solves (FP64 OpenMP oracle), {el:.1f} s")
+    return dict(value=n_sample * blocks * SC_STEP / el, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"first {n_sample} of the {n_total} particles x {blocks * SC_STEP} Boris steps with "
+                       f"{blocks} SC solves on the {GRID[0]}x{GRID[1]}x{GRID[2]} grid (FP64 OpenMP oracle), "
+                       f"{el:.1f} s")
 
 
 def surrogate_leg(M, tmp, warmup, steps, train_iters):
     """Train the network on the GPU, then run the FP32 tier with it against
-    the FP32 and FP64 order-4 map tiers on the headline workload."""
+    the FP32 and FP64 order-4 map tiers (both azimuth-ordered) on the
+    headline workload."""
     import torch
     from pyffa_amd.FFAG_Surrogate import FFAG_FieldSurrogate
-    _, gp4 = machines(4, tmp)
+    gp4 = machines(4, tmp)
     sur = FFAG_FieldSurrogate(gp4, zt=0.01)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -219,33 +240,20 @@ def surrogate_leg(M, tmp, warmup, steps, train_iters):
            "loss_history": hist, "field_error": err, "zt_m": sur.zt,
            "mfma": {"kernel": "k_mlp_eval (4M points)", "ms": t_eval * 1e3, "achieved_tflops": tflops,
                     "peak_tflops": BF16_PEAK_TFLOPS, "frac": tflops / BF16_PEAK_TFLOPS}}
-    ps = M.shape[0] * steps * SC_STEP
+    ps = M.shape[0] * steps * PUSHES
     B, rk, el_sur, S_sur = run_tier("fp32", M, gp4, warmup, steps, surrogate=sur)
     del B, rk
     B, rk, el_map32, S_map32 = run_tier("fp32", M, gp4, warmup, steps)
     del B, rk
-    B, rk, el_map64, S_map64 = run_tier("fp64", M, gp4, warmup, steps)
+    B, rk, el_map64, S_map64 = run_tier("fp64", M, gp4, warmup, steps, sort_every=PUSHES)
     del B, rk
-    after = (warmup + steps) * SC_STEP
+    after = (warmup + steps) * PUSHES
     out.update({"value": ps / el_sur, "map_order4_fp32_value": ps / el_map32, "map_order4_fp64_value": ps / el_map64,
                 "speedup_vs_fp64_order4": el_map64 / el_sur,
                 "accuracy_vs_fp64_order4": accuracy(S_sur, S_map64, after),
                 "fp32_map_accuracy_vs_fp64_order4": accuracy(S_map32, S_map64, after)})
     log(f"surrogate tier {ps / el_sur:.3e}, fp32 order-4 {ps / el_map32:.3e}, fp64 order-4 {ps / el_map64:.3e}")
     return out
-
-
-def scale_leg(tmp, n, steps):
-    """FP32 tier vs FP64 path at 8M particles on the (512, 256, 256) grid."""
-    from tests import cases
-    _, gp = machines(2, tmp)
-    M = cases.bunch("stable", n=n, seed=0)
-    ps = n * steps * SC_STEP
-    _, _, el32, S32 = run_tier("fp32", M, gp, 1, steps, grid=GRID_8M)
-    _, _, el64, S64 = run_tier("fp64", M, gp, 1, steps, grid=GRID_8M)
-    return {"particles": n, "sc_grid": list(GRID_8M[:3]), "steps": steps, "value_fp32": ps / el32,
-            "value_fp64": ps / el64, "speedup_fp32_vs_fp64": el64 / el32,
-            "accuracy_fp32_vs_fp64": accuracy(S32, S64, (1 + steps) * SC_STEP)}
 
 
 def orbit_moments(S, seo):
@@ -296,10 +304,10 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
     import numpy as np
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
-    from tests import cases
-    _, gp = machines(2, tmp)
-    seo = cases.seo("stable")
-    M0 = cases.bunch("stable", n=n, seed=3)
+    from pyffa_amd import workloads as W
+    gp = machines(2, tmp)
+    seo = W.seo("stable")
+    M0 = W.bunch("stable", n=n, seed=3)
     steps_per_turn = seo.T0[3] / DT
     total = int(round(turns * steps_per_turn / SC_STEP))
     n_chunks = max(1, int(round(turns / chunk_turns)))
@@ -312,7 +320,7 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
             M[:, 0] = np.nextafter(M[:, 0], np.inf)
             M[:, 2] = np.nextafter(M[:, 2], np.inf)
         B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=grid, BunchType="Boris", precision=prec)
-        cases.set_bunch_seo(B)
+        W.set_bunch_seo(B)
         rk = FFAG_RungeKutta()
         t, done, hist = 0.0, 0, []
         t0 = time.perf_counter()
@@ -357,8 +365,8 @@ def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
     import torch
     from numpy.polynomial import Polynomial
     from pyffa_amd import FFAG_HighOrder as H
-    from tests import cases
-    cfg = cases.lattice("stable").to_config()["Bmap"]
+    from pyffa_amd import workloads as W
+    cfg = W.lattice("stable").to_config()["Bmap"]
     r, fi = H.grid_from_config(cfg)
     s_fit = Polynomial.fit(r, H.calculate_s(r, 0.0, 35.0), 5)
     conv = H.BmapParams_to_ConvexParams(cfg)
@@ -401,7 +409,7 @@ def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
     return out
 
 
-def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False):
+def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False, grid=GRID):
     """Incoherent tunes with space charge: the bunch of ``accuracy_leg`` is
     tracked ``turns`` + 1.5 turns in each run ("bare": FP64 without SC;
     "fp64", "fp32": with SC; "fp64p": FP64 with SC from r and z moved by one
@@ -415,13 +423,13 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_dump import Dumps, PositionDump
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
-    from tests import cases
-    _, gp = machines(2, tmp)
-    seo = cases.seo("stable")
-    M0 = cases.bunch("stable", n=n, seed=3)
+    from pyffa_amd import workloads as W
+    gp = machines(2, tmp)
+    seo = W.seo("stable")
+    M0 = W.bunch("stable", n=n, seed=3)
     steps_per_turn = seo.T0[3] / DT
     total = int(round((turns + 1.5) * steps_per_turn / SC_STEP)) * SC_STEP
-    out = {"particles": n, "turns": turns, "tracked": n_track, "sc_grid": list(GRID[:3]), "steps": total,
+    out = {"particles": n, "turns": turns, "tracked": n_track, "sc_grid": list(grid[:3]), "steps": total,
            "estimator": "FFT of x + i x'/scale, 8x zero padding, parabolic peak; x about r0(Ek, phi)"}
     q = {}
     for run in runs:
@@ -430,8 +438,8 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
         if run == "fp64p":
             M[:, 0] = np.nextafter(M[:, 0], np.inf)
             M[:, 2] = np.nextafter(M[:, 2], np.inf)
-        B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=GRID, BunchType="Boris", precision=prec)
-        cases.set_bunch_seo(B)
+        B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=grid, BunchType="Boris", precision=prec)
+        W.set_bunch_seo(B)
         folder = os.path.join(tmp, f"tune_{run}")
         pd = PositionDump(0.0, -1.0, 1.0, np.arange(n_track), save_folder=folder, max_records=turns + 8)
         dumps = Dumps()
@@ -481,25 +489,43 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
     return out
 
 
+def field_error_leg(gp, n=1_000_000, seed=11):
+    """Relative error of the FP32 tier's field (its own FP32 evaluation on
+    the FP32 map) against the FP64 path's field at n points uniform over the
+    map (r 8.0-9.3 m, all azimuths, |z| <= 5 cm)."""
+    import numpy as np
+    from pyffa_amd.FFAG_Field import field_at_points
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(8.0, 9.3, n)
+    fi = rng.uniform(0.0, 2 * np.pi, n)
+    z = rng.uniform(-0.05, 0.05, n)
+    b64 = field_at_points(gp, r, fi, z, precision="fp64")
+    b32 = field_at_points(gp, r, fi, z, precision="fp32")
+    rms = float(np.sqrt((b64 ** 2).sum(axis=0).mean()))
+    rel = np.sqrt(((b32 - b64) ** 2).sum(axis=0)) / rms
+    return {"points": n, "region": "r 8.0-9.3 m, all azimuths, |z| <= 5 cm", "rms_B": rms,
+            "rel_rms": float(np.sqrt(np.mean(rel ** 2))), "rel_p99": float(np.quantile(rel, 0.99)),
+            "rel_max": float(rel.max()),
+            "measure": "|B_fp32 - B_fp64| / rms |B_fp64| per point (Bz, Bphi, Br), as the surrogate's field error"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300, help="timed bench steps (20 pushes + 1 SC solve each)")
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20, help=f"timed bench steps ({PUSHES} pushes each)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=N_PART)
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=10, help="SC blocks of 20 steps in the CPU baseline sample")
+    ap.add_argument("--cpu-blocks", type=int, default=3, help="SC blocks of 20 steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="timed bench steps of the FP64 legs (default --steps)")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--surrogate-steps", type=int, default=50)
+    ap.add_argument("--surrogate-steps", type=int, default=2)
     ap.add_argument("--train-iters", type=int, default=12000)
-    ap.add_argument("--no-scale", action="store_true")
-    ap.add_argument("--scale-particles", type=int, default=8_000_000)
-    ap.add_argument("--scale-steps", type=int, default=10)
     ap.add_argument("--accuracy-turns", type=float, default=0.0,
                     help="also run both tiers this many turns and report the moments' deviation (off by default)")
-    ap.add_argument("--accuracy-particles", type=int, default=100_000)
+    ap.add_argument("--accuracy-particles", type=int, default=N_PART)
     ap.add_argument("--accuracy-tiers", default="fp32,fp64,fp64p")
     ap.add_argument("--accuracy-grid", default=None, help="SC grid ns,nx,nz of the accuracy leg")
     ap.add_argument("--accuracy-chunk", type=float, default=50.0, help="turns between moment samples")
@@ -508,8 +534,9 @@ def main():
     ap.add_argument("--only-mapgen", action="store_true", help="run the Taylor-map generator leg alone")
     ap.add_argument("--tune-turns", type=int, default=0,
                     help="also measure incoherent tune shifts over this many turns (off by default)")
-    ap.add_argument("--tune-particles", type=int, default=100_000)
-    ap.add_argument("--tune-tracked", type=int, default=256)
+    ap.add_argument("--tune-particles", type=int, default=N_PART)
+    ap.add_argument("--tune-tracked", type=int, default=2000)
+    ap.add_argument("--tune-grid", default=None, help="SC grid ns,nx,nz of the tune leg")
     ap.add_argument("--only-tune", action="store_true", help="run the tune leg alone")
     ap.add_argument("--tune-progress", action="store_true",
                     help="the tune leg's tracking prints its progress lines (on stdout, before the JSON line)")
@@ -533,7 +560,7 @@ def main():
         return
     if args.only_tune:
         print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
-                                                 progress=args.tune_progress)}),
+                                                 progress=args.tune_progress, grid=_grid_arg(args.tune_grid))}),
               flush=True)
         return
     if args.only_accuracy:
@@ -541,42 +568,55 @@ def main():
                            chunk_turns=args.accuracy_chunk, grid=_grid_arg(args.accuracy_grid))
         print(json.dumps({"accuracy_long": acc}), flush=True)
         return
-    from tests import cases
-    mc, gp = machines(2, tmp)
-    M = cases.bunch("stable", n=args.particles, seed=0)
-    log(f"inputs ready: {M.shape[0]} particles")
+    from pyffa_amd import workloads as W
+    gp = machines(2, tmp)
+    M = W.bunch("stable", n=args.particles, seed=0)
+    n = M.shape[0]
+    log(f"inputs ready: {n} particles")
     B32, rk32, el32, S32 = run_tier("fp32", M, gp, args.warmup, args.steps)
     k32 = kernel_time_ms(rk32, B32, gp, "fp32")
     log(f"fp32 tier {el32:.3f} s, kernels {k32}")
     del B32, rk32
-    n_alive = M.shape[0]
-    ps = n_alive * args.steps * SC_STEP
+    ps = n * args.steps * PUSHES
     out = {
         "metric": "particle-steps/s w/ 2.5D SC; surrogate+FP32 vs FP64 map gather at fixed error",
         "value": ps / el32, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el32 / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 state / f32 field", "data": "synthetic",
-        "config": {"workload": "1M protons (1e13 total), 12-sector scaling FFAG r0=8.5 m k=7.6 "
+        "config": {"workload": f"{n / 1e6:g}M protons (1e13 total), 12-sector scaling FFAG r0=8.5 m k=7.6 "
                                "(D=-1.4 vertically stable variant), order-2 Taylor maps, aperture r 8.0-9.3 m "
-                               "|z|<5 cm, 2.5D SC (ns,nx,nz)=(256,128,128) every 20 steps, 2 RF gaps 100 kV, "
-                               "Boris dt=0.05 ns; FP32-field tier",
-                   "particles": n_alive, "sc_grid": list(GRID[:3]), "sc_step": SC_STEP},
+                               "|z|<5 cm, 2.5D SC (ns,nx,nz)=(512,256,256) every 20 steps, 2 RF gaps 100 kV, "
+                               "Boris dt=0.05 ns; FP32-field tier; one step = 1000 pushes, 50 SC solves, "
+                               "1 azimuth sort, 1 stop check",
+                   "particles": n, "sc_grid": list(GRID[:3]), "sc_step": SC_STEP, "pushes_per_step": PUSHES},
     }
     if not args.no_fp64:
-        B64, rk64, el64, S64 = run_tier("fp64", M, gp, args.warmup, args.steps)
+        k = args.fp64_steps or args.steps
+        ps64 = n * k * PUSHES
+        B64, rk64, el64, S64 = run_tier("fp64", M, gp, args.warmup, k)
         k64 = kernel_time_ms(rk64, B64, gp, "fp64")
         del B64, rk64
-        log(f"fp64 path {el64:.3f} s, kernels {k64}")
-        out["fp64_value"] = ps / el64
-        out["speedup_fp32_vs_fp64"] = el64 / el32
-        out["accuracy_fp32_vs_fp64"] = accuracy(S32, S64, (args.warmup + args.steps) * SC_STEP)
+        log(f"fp64 path (input order) {el64:.3f} s, kernels {k64}")
+        B64s, rk64s, el64s, _ = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES)
+        k64s = kernel_time_ms(rk64s, B64s, gp, "fp64")
+        del B64s, rk64s
+        log(f"fp64 path (azimuth-ordered) {el64s:.3f} s, kernels {k64s}")
+        out["fp64_value"] = ps64 / el64
+        out["fp64_sorted_value"] = ps64 / el64s
+        out["speedup_fp32_vs_fp64"] = (ps / el32) / (ps64 / el64)
+        out["speedup_fp32_vs_fp64_sorted"] = (ps / el32) / (ps64 / el64s)
         out["fp64_kernel_ms"] = k64
+        out["fp64_sorted_kernel_ms"] = k64s
+        if k == args.steps:
+            out["accuracy_fp32_vs_fp64"] = accuracy(S32, S64, (args.warmup + args.steps) * PUSHES)
+        out["field_error_fp32_vs_fp64"] = field_error_leg(gp)
+        log(f"field error fp32 vs fp64 {out['field_error_fp32_vs_fp64']}")
     # roofline of the dominant kernel of the headline tier: the fused 20-step
     # push.  Algorithmic HBM bytes per particle per launch: FP64 state read +
     # write (7 x 8 x 2), flag byte read + write, FP32 SC endpoints Pre/PrePre
     # (6 x 4).
     bytes_per_particle = 7 * 8 * 2 + 2 + 6 * 4
-    algo = n_alive * bytes_per_particle
+    algo = n * bytes_per_particle
     ach = algo / (k32["push"] * 1e-3) / 1e9
     # the fused push keeps the state in registers for 20 steps, so it is
     # bound by vector-ALU issue rather than HBM: the measured VALU busy
@@ -587,21 +627,18 @@ def main():
                        "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"],
                        "valu": measured_valu("k_boris_fused<float, false>")}
     if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(M, tmp, 2, args.surrogate_steps, args.train_iters)
-    if not args.no_scale:
-        out["scale_8m"] = scale_leg(tmp, args.scale_particles, args.scale_steps)
-        log(f"8M leg {out['scale_8m']}")
+        out["surrogate"] = surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters)
     if not args.no_mapgen:
         out["map_generator"] = mapgen_leg(cpu=not args.no_cpu)
     if args.tune_turns > 0:
         out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
-                                     progress=args.tune_progress)
+                                     progress=args.tune_progress, grid=_grid_arg(args.tune_grid))
     if args.accuracy_turns > 0:
         out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
                                             args.accuracy_tiers.split(","), chunk_turns=args.accuracy_chunk,
                                             grid=_grid_arg(args.accuracy_grid))
     if not args.no_cpu:
-        cb = cpu_baseline(M, mc, min(args.cpu_particles, M.shape[0]), args.cpu_steps)
+        cb = cpu_baseline(M, gp, min(args.cpu_particles, n), args.cpu_blocks, n)
         log(f"cpu baseline {cb}")
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

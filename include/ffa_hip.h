@@ -180,6 +180,9 @@ typedef struct {
 int ffa_create(int device, void *stream, void **ctx_out);
 int ffa_destroy(void *ctx);
 const char *ffa_last_error(void *ctx);
+/* Later calls run on stream; the new stream first waits for the work already
+ * queued on the old one (an event), so a call that rewrites the context's
+ * device tables cannot overtake kernels still reading them. */
 int ffa_set_stream(void *ctx, void *stream);
 
 /* FFAG_Bfield_analytical / FFAG_EField_new / FFAG_BField_Error /
@@ -273,8 +276,12 @@ int ffa_sort_particles(void *ctx, const ffa_particles *src, const ffa_particles 
  *   mode 2: that part in coefficient form for a trained range zt:
  *           (sum_n Bz_n z^(2n-2) zt^2, sum_n Bfi_n z^(2n-2) zt^3,
  *            sum_n Br_n z^(2n-2) zt^3), i.e. mode 1 divided by (z/zt)^2,
- *           (z/zt)^3, (z/zt)^3.
- * Used to label the surrogate's training samples and to judge its field. */
+ *           (z/zt)^3, (z/zt)^3;
+ *   mode 3: the mode-0 field as the FP32 tier's push evaluates it (FP32
+ *           arithmetic on the FP32 Taylor map), widened to double; needs a
+ *           machine whose Taylor map is float (taylor_f32 = 1).
+ * Used to label the surrogate's training samples and to judge its field and
+ * the FP32 tier's field against the FP64 one. */
 int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *z, int64_t n, double *out,
                     int32_t mode, double zt);
 

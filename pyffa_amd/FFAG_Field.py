@@ -309,6 +309,34 @@ def mean_order0_bz(bm, r, theta):
     return bm.B0 * r ** bm.k_value * F.mean(axis=1)
 
 
+def field_at_points(gp, r, fi, z, precision="fp64", device=0):
+    """Static magnetic field (Bz, Bfi, Br) at host points, as a (3, n) numpy
+    array, evaluated on the GPU the way a tracking tier evaluates it:
+    "fp64" is the FP64 path's field (GetBfields_Njit + Bmap_add_all_orders +
+    harmonics, src/FFAG_Field.py:872-884, src/FFAG_MathTools.py:1139-1188;
+    ffa_bfield_eval mode 0), "fp32" the FP32 tier's (same terms in FP32 on the
+    FP32 map; mode 3)."""
+    import torch
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be 'fp64' or 'fp32'")
+    dev = torch.device("cuda", device)
+    ctx = _lib.Context(device, torch.cuda.current_stream(dev).cuda_stream)
+    f32 = precision == "fp32"
+    tay = gp.Bmap.device_taylor(torch.float32 if f32 else torch.float64, dev)
+    ctx.check(ctx.lib.ffa_set_machine(ctx.h, build_machine(gp, tay, f32)), "ffa_set_machine")
+    cols = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev) for a in (r, fi, z)]
+    n = cols[0].numel()
+    if cols[1].numel() != n or cols[2].numel() != n:
+        raise ValueError("r, fi and z must have the same length")
+    out = torch.empty(3 * max(n, 1), dtype=torch.float64, device=dev)
+    ctx.check(ctx.lib.ffa_bfield_eval(ctx.h, _lib._vp(cols[0].data_ptr()), _lib._vp(cols[1].data_ptr()),
+                                      _lib._vp(cols[2].data_ptr()), n, _lib._vp(out.data_ptr()), 3 if f32 else 0,
+                                      0.0), "ffa_bfield_eval")
+    res = out[: 3 * n].view(3, n).cpu().numpy()
+    ctx.close()
+    return res
+
+
 def build_machine(gp, taylor_tensor, taylor_f32: bool) -> "_lib.FfaMachine":
     """Pack FFAG_GlobalParameters into the C-ABI machine description."""
     bm = gp.Bmap

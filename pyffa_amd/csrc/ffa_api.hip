@@ -91,7 +91,21 @@ const char *ffa_last_error(void *ctx) {
 int ffa_set_stream(void *ctx, void *stream) {
     ffa_ctx_s *c = as_ctx(ctx);
     if (!c) return fail(nullptr, FFA_ERR_INVALID, "ctx is NULL");
-    c->stream = reinterpret_cast<hipStream_t>(stream);
+    hipStream_t ns = reinterpret_cast<hipStream_t>(stream);
+    if (ns != c->stream) {
+        // work already queued on the old stream may still read the context's
+        // device block (machine, step tables): the new stream waits for it
+        hipEvent_t ev;
+        hipError_t e = hipSetDevice(c->device);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_check(c, e, "ffa_set_stream");
+        e = hipEventRecord(ev, c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ns, ev, 0);
+        const hipError_t e2 = hipEventDestroy(ev);
+        if (e == hipSuccess) e = e2;
+        if (e != hipSuccess) return hip_check(c, e, "ffa_set_stream");
+    }
+    c->stream = ns;
     return FFA_OK;
 }
 
@@ -333,11 +347,16 @@ int ffa_bfield_eval(void *ctx, const double *r, const double *fi, const double *
     if (!c) return fail(c, FFA_ERR_INVALID, "ctx is NULL");
     if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
     if (n < 0) return fail(c, FFA_ERR_INVALID, "negative point count");
-    if (mode < 0 || mode > 2) return fail(c, FFA_ERR_INVALID, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(c, FFA_ERR_INVALID, "mode must be 0, 1, 2 or 3");
     if (mode == 2 && !(zt > 0.0)) return fail(c, FFA_ERR_INVALID, "mode 2 needs zt > 0");
     if (n > 0 && (!r || !fi || !z || !out)) return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (mode == 3) {
+        if (c->host_machine.max_order > 0 && !c->host_machine.taylor_f32)
+            return fail(c, FFA_ERR_INVALID, "ffa_bfield_eval mode 3 needs the FP32 Taylor map");
+        return hip_check(c, launch_bfield_eval_f32(c, r, fi, z, n, out), "ffa_bfield_eval");
+    }
     if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
-        return fail(c, FFA_ERR_INVALID, "ffa_bfield_eval needs the FP64 Taylor map");
+        return fail(c, FFA_ERR_INVALID, "ffa_bfield_eval modes 0-2 need the FP64 Taylor map");
     return hip_check(c, launch_bfield_eval(c, r, fi, z, n, out, mode, zt), "ffa_bfield_eval");
 }
 

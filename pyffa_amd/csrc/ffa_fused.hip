@@ -360,7 +360,34 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     p.fidot[i] = (-vx * sn + vy * cn) * irn;
 }
 
+// ffa_bfield_eval mode 3: the static field exactly as the FP32 tier's push
+// evaluates it (fields_f32 on the FP32 map), widened to double.  The azimuth
+// is reduced to (-pi, pi] in FP64 first, the range the push's atan2 gives.
+__global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__restrict__ kc,
+                                                           const double *__restrict__ r, const double *__restrict__ fi,
+                                                           const double *__restrict__ z, int64_t n,
+                                                           double *__restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double f = fi[i] - FFA_TWO_PI * floor(fi[i] * (1.0 / FFA_TWO_PI));
+    if (f > FFA_PI) f -= FFA_TWO_PI;
+    const float rr = (float)r[i], th = (float)f, zf = (float)z[i];
+    float Bz, Br, Bfi, Er, Efi, Ez, srf;
+    fields_f32(kc, rr, th, zf, rr * __cosf(th), rr * __sinf(th), 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
+    out[i] = (double)Bz;
+    out[n + i] = (double)Bfi;
+    out[2 * n + i] = (double)Br;
+}
+
 }  // namespace
+
+hipError_t launch_bfield_eval_f32(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
+                                  double *out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bfield_eval_f32, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
+                       ctx_fconst_dev(c), r, fi, z, n, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps) {
     if (p->n == 0 || nsteps == 0) return hipSuccess;

@@ -95,6 +95,8 @@ hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t 
 hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_dev);
 hipError_t launch_bfield_eval(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
                               double *out, int mode, double zt);
+hipError_t launch_bfield_eval_f32(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
+                                  double *out);
 hipError_t launch_mlp_pack(ffa_ctx_s *c, const float *params, void *packed);
 hipError_t launch_mlp_grad(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *packed, const float *pts,
                            const float *tgt, int64_t n, void *ws, float *params, float *grad, float *mom, float *vel,
