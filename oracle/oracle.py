@@ -53,6 +53,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         _lib = ctypes.CDLL(LIB_PATH)
+        _lib.ora_sc_local_coords.restype = ctypes.c_int64
     return _lib
 
 
