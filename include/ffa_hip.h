@@ -240,6 +240,81 @@ int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_s
  * ffa_sc_workspace_layout. */
 int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace);
 
+/* ---- Stage-level space charge: the module functions of FFAG_SC_new -------
+ * Each entry point below runs one stage of the 2.5D solve on device arrays
+ * laid out as the reference's numpy arrays (the (N, 16) row-major particle
+ * matrix, (ns, stride) per-slice lists, (ns, nz, nx) float32 cubes), for
+ * callers that drive the stages themselves (src/FFAG_Bunch.py:1157-1352
+ * calls them in this order).  Entry points with a workspace take a device
+ * block of at least ffa_sc_stage_workspace_size(n, ns, nz, nx) bytes for the
+ * largest n and grid they are called with, and its size ws_bytes.  Sums are
+ * fixed-order tree reductions (reproducible run to run; not the
+ * reference's sequential rounding). */
+int ffa_sc_stage_workspace_size(int64_t n, int32_t ns, int32_t nz, int32_t nx, size_t *bytes);
+
+/* GetLocalCoordinates_2p5 / _rk (src/FFAG_SC_new.py:74-148): the rows with
+ * survive[i] != 0, in row order, into out = 6 planes of n doubles (r, fi, z,
+ * gamma, fi mod 2 pi, row index); count_sum (device, 2 doubles) = (count,
+ * sum of gamma).  gamma = 1/sqrt(1 - v^2/c^2) from columns 0-5 (rk = 0) or
+ * E_total / E0 (rk != 0). */
+int ffa_sc_local_coords(void *ctx, const double *bunch, const uint8_t *survive, int64_t n, double speed_c, int32_t rk,
+                        double *out, double *count_sum, void *workspace, size_t ws_bytes);
+
+/* divide_arc2 (src/FFAG_SC_new.py:534-566): out (device, 3 doubles) =
+ * (arc start, arc end, span) of the smallest of the four shifted covers. */
+int ffa_sc_divide_arc(void *ctx, const double *fi_mod, int64_t n, double *out, void *workspace, size_t ws_bytes);
+
+/* ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531):
+ * x = r - r0, slice s = int((fi_mod (+2 pi if below fi_s) - fi_s) / slice_step)
+ * clamped to [0, ns); counts[ns], slice_idx[n], sums (device, 4 doubles) =
+ * (sum x, sum z, sum x^2, sum z^2); the per-slice lists r2d (may be NULL),
+ * z2d, x2d (ns rows of `stride` doubles) in particle order. */
+int ffa_sc_slices(void *ctx, const double *r, const double *z, const double *fi_mod, const double *r0, int64_t n,
+                  double slice_step, double fi_s, int32_t ns, double *x, double *r2d, double *z2d, double *x2d,
+                  int64_t stride, int32_t *counts, int32_t *slice_idx, double *sums, void *workspace,
+                  size_t ws_bytes);
+
+/* DistributeChargeSBS_Njit (src/FFAG_SC_new.py:244-312) on the per-slice
+ * lists: the reference's CIC cells and clamped fractions, rounded to `bits`
+ * fractional bits and summed as integers (bit-reproducible), scaled by
+ * q_macro ns / (bunch_length dx dz) into rho (ns, nz, nx) float32; zeros
+ * when dx, dz or bunch_length is not positive. */
+int ffa_sc_deposit(void *ctx, const double *z2d, const double *x2d, int64_t stride, const int32_t *counts, int32_t ns,
+                   int32_t nz, int32_t nx, double q_macro, double bunch_length, double xmin, double xmax, double zmin,
+                   double zmax, int32_t bits, float *rho, void *workspace, size_t ws_bytes);
+
+/* make_fft_kernel_open_numba (src/FFAG_SC_new.py:731-760): the periodic
+ * Green's function 1 / (eps0 k^2), (nz, nx) float32, k = 0 mode zero. */
+int ffa_sc_green(void *ctx, int32_t nx, int32_t nz, double dx, double dz, float *kernel, void *workspace,
+                 size_t ws_bytes);
+
+/* The per-slice periodic Poisson solve of Build_2_5D_SliceGrid_sbs
+ * (src/FFAG_Bunch.py:1330-1338, pyFFTW plans of src/FFAG_track.py:318-328):
+ * phi_k = Re IFFT2(FFT2(rho_k) G) / (nz nx) for every slice, on the LDS
+ * Stockham FFT (nx, nz in [32, 1024] with no prime factor above 5). */
+int ffa_sc_poisson(void *ctx, const float *rho, int32_t ns, int32_t nz, int32_t nx, double dx, double dz, float *phi,
+                   void *workspace, size_t ws_bytes);
+
+/* compute_field_from_potential_numba_SBS (src/FFAG_SC_new.py:672-728) for ns
+ * slices of phi: ex, ez (ns, nz, nx) float32 = -grad(phi) / gamma^2 with the
+ * reference's edge stencils; zeros when dr, dz or bunch_length is 0. */
+int ffa_sc_field(void *ctx, const float *phi, int32_t ns, int32_t nz, int32_t nx, double dr, double dz,
+                 double bunch_length, double mean_gamma_m1, float *ex, float *ez, void *workspace, size_t ws_bytes);
+
+/* total_field_1d_numba (src/FFAG_SC_new.py:763-782): E[i] = K sum_{j != i}
+ * q_j sign(s_i - s_j) / (s_i - s_j)^2, zeros unless s[1] > s[0]. */
+int ffa_sc_total_field_1d(void *ctx, const double *s, const double *q, int32_t n, double *E);
+
+/* interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972):
+ * bilinear ex, ez of each particle's slice at (x, z), projected with pr0,
+ * into ex_p, ez_p, ef_p and, when bunch is given, columns 10-12 of row
+ * rows[k] of the (n_rows, 16) matrix; zeros when dx, dz or bunch_length is
+ * not positive. */
+int ffa_sc_interpolate(void *ctx, const float *ex, const float *ez, int32_t ns, int32_t nz, int32_t nx, const double *x,
+                       const double *z, const double *pr0, const int32_t *slice, const int64_t *rows, int64_t n,
+                       double x0, double z0, double dx, double dz, double bunch_length, double *ex_p, double *ez_p,
+                       double *ef_p, double *bunch, int64_t n_rows);
+
 /* Stop-condition reduction of src/FFAG_track.py:559-577: out (device, 2
  * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);

@@ -485,23 +485,39 @@ class FFAG_Bunch:
             self.sc_solve_device(ctx, first=first, s_last=s_last)
         finally:
             self.sc_field_cubes, self.sc_charge_grid = keep
+        from . import FFAG_SC_new as SC
         res = self.sc_result()
+        ns = self.f_grid_size
+        bad = res["bad_grid"] != 0.0
         rho = self.sc_view("grid").cpu().numpy().astype(np.float64) * (
             res["qscale"] / float(1 << (2 * fixed_point_bits(max(self.TotalParticleNum_Local, 1)))))
-        phi = self.sc_view("phi").cpu().numpy()
-        ex = self.sc_view("ex").cpu().numpy()
-        ez = self.sc_view("ez").cpu().numpy()
-        ns = self.f_grid_size
         fi_grid = np.linspace(res["fi_s"], res["fi_e"], ns)
         s_grid = fi_grid / np.pi / 2.0 * res["perimeter"]
         counts = self.sc_view("slice_count").cpu().numpy()
         rho_f = counts * (self.marcosize * self.q_charge)
-        ef = _total_field_1d(s_grid, rho_f)
+        if bad:
+            # degenerate box: the reference returns zero cubes and a zero
+            # longitudinal field, and leaves the particles' fields as they were
+            shape = (ns, self.z_grid_size, self.x_grid_size)
+            phi, ex, ez = (np.zeros(shape, np.float32) for _ in range(3))
+            ef = np.zeros(ns)
+        else:
+            phi = self.sc_view("phi").cpu().numpy()
+            ex = self.sc_view("ex").cpu().numpy()
+            ez = self.sc_view("ez").cpu().numpy()
+            ef = SC.total_field_1d_numba(s_grid, rho_f)
+        # survivors of this solve (its alive flags) in the input order; the
+        # solve's x = r - r0 is held in device order, so it is put back in
+        # the input order through the particle ids first
+        t = self.torch
+        alive_dev = (self.flags & _lib.F_ALIVE) != 0
+        alive = self.ordered("flags").cpu().numpy() & _lib.F_ALIVE != 0
+        px_in = t.zeros_like(self.r)
+        px_in[self.pid[alive_dev]] = self.sc_view("px")[alive_dev]
+        px = px_in.cpu().numpy()[alive]
         M = self.LocalBunch
-        alive = (M[:, 8] != 0)
-        px = self.sc_view("px").cpu().numpy()[alive]
         return (rho.astype(np.float32), phi, ex, ez, res["xmin"], res["xmax"], res["zmin"], res["zmax"],
-                res["bunch_length"], ef, s_grid, rho_f, M[alive, 0], np.mod(M[alive, 4], 2 * np.pi), px,
+                res["bunch_length"], ef, s_grid, rho_f, M[alive, 0], M[alive, 4], px,
                 M[alive, 0] - px, M[alive, 2], fi_grid)
 
     def sc_solve_device(self, ctx=None, *, first, s_last):
@@ -518,17 +534,3 @@ class FFAG_Bunch:
 
 def ctypes_len(ty):
     return getattr(ty, "_length_", 1)
-
-
-def _total_field_1d(s, q):
-    """Longitudinal Coulomb sum over slice charges (src/FFAG_SC_new.py:763-782);
-    computed for the returned tuple only -- the reference does not apply it."""
-    eps0 = 8.854187817e-12
-    K = 1.0 / (4.0 * np.pi * eps0)
-    n = s.size
-    if n < 2 or s[1] - s[0] <= 0.0:
-        return np.zeros(n)
-    d = s[:, None] - s[None, :]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        w = np.where(d != 0, np.sign(d) / (d * d), 0.0)
-    return K * (w @ q)

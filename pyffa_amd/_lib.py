@@ -144,7 +144,10 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
             "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing",
-            "ffa_taylor_maps", "ffa_orbit_track", "ffa_spiral_gap_maps")
+            "ffa_taylor_maps", "ffa_orbit_track", "ffa_spiral_gap_maps",
+            "ffa_sc_stage_workspace_size", "ffa_sc_local_coords", "ffa_sc_divide_arc", "ffa_sc_slices",
+            "ffa_sc_deposit", "ffa_sc_green", "ffa_sc_poisson", "ffa_sc_field", "ffa_sc_total_field_1d",
+            "ffa_sc_interpolate")
 
 
 def load():
@@ -188,6 +191,20 @@ def load():
     lib.ffa_taylor_maps.argtypes = [_vp, ctypes.POINTER(FfaTaylorGen)]
     lib.ffa_orbit_track.argtypes = [_vp, ctypes.POINTER(FfaOrbit)]
     lib.ffa_spiral_gap_maps.argtypes = [_vp, ctypes.POINTER(FfaSpiralGap)]
+    _sz = ctypes.c_size_t
+    lib.ffa_sc_stage_workspace_size.argtypes = [_i64, _i32, _i32, _i32, ctypes.POINTER(_sz)]
+    lib.ffa_sc_local_coords.argtypes = [_vp, _vp, _vp, _i64, _d, _i32, _vp, _vp, _vp, _sz]
+    lib.ffa_sc_divide_arc.argtypes = [_vp, _vp, _i64, _vp, _vp, _sz]
+    lib.ffa_sc_slices.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                  _vp, _vp, _sz]
+    lib.ffa_sc_deposit.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32, _d, _d, _d, _d, _d, _d, _i32, _vp,
+                                   _vp, _sz]
+    lib.ffa_sc_green.argtypes = [_vp, _i32, _i32, _d, _d, _vp, _vp, _sz]
+    lib.ffa_sc_poisson.argtypes = [_vp, _vp, _i32, _i32, _i32, _d, _d, _vp, _vp, _sz]
+    lib.ffa_sc_field.argtypes = [_vp, _vp, _i32, _i32, _i32, _d, _d, _d, _d, _vp, _vp, _vp, _sz]
+    lib.ffa_sc_total_field_1d.argtypes = [_vp, _vp, _vp, _i32, _vp]
+    lib.ffa_sc_interpolate.argtypes = [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _d,
+                                       _d, _vp, _vp, _vp, _vp, _i64]
     for name in EXPORTED:
         if name != "ffa_last_error":
             getattr(lib, name).restype = ctypes.c_int

@@ -30,7 +30,14 @@ int hip_check(ffa_ctx_s *c, hipError_t e, const char *what) {
     return fail(c, FFA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+// transverse SC grid sizes the LDS FFT handles: 32..1024 points whose only
+// prime factors are 2, 3 and 5
+bool fft_size_ok(int v) {
+    if (v < 32 || v > 1024) return false;
+    for (int f : {2, 3, 5})
+        while (v % f == 0) v /= f;
+    return v == 1;
+}
 
 int check_particles(ffa_ctx_s *c, const ffa_particles *p) {
     if (!p) return fail(c, FFA_ERR_INVALID, "particles is NULL");
@@ -228,8 +235,9 @@ int ffa_push_boris_fused_f32(void *ctx, const ffa_particles *p, const ffa_step *
 int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *out) {
     if (!out) return fail(nullptr, FFA_ERR_INVALID, "out is NULL");
     if (n < 0 || ns < 2 || ns > 4096) return fail(nullptr, FFA_ERR_INVALID, "need n >= 0 and 2 <= ns <= 4096");
-    if (!pow2_in(nx, 32, 1024) || !pow2_in(nz, 32, 1024))
-        return fail(nullptr, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", nx, nz);
+    if (!fft_size_ok(nx) || !fft_size_ok(nz))
+        return fail(nullptr, FFA_ERR_INVALID, "nx, nz must be in [32, 1024] with no prime factor above 5 (got %d, %d)",
+                    nx, nz);
     return ffa_sc_layout_compute(n, ns, nz, nx, out);
 }
 
@@ -239,8 +247,9 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
     if (!(sp->c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed of light must be positive");
     if (int rc = check_particles(c, p)) return rc;
     if (sp->ns < 2 || sp->ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [2, 4096]");
-    if (!pow2_in(sp->nx, 32, 1024) || !pow2_in(sp->nz, 32, 1024))
-        return fail(c, FFA_ERR_INVALID, "nx, nz must be powers of two in [32, 1024] (got %d, %d)", sp->nx, sp->nz);
+    if (!fft_size_ok(sp->nx) || !fft_size_ok(sp->nz))
+        return fail(c, FFA_ERR_INVALID, "nx, nz must be in [32, 1024] with no prime factor above 5 (got %d, %d)",
+                    sp->nx, sp->nz);
     if (sp->bits < 1 || sp->bits > 20) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 20]");
     if ((double)p->n * (double)(1LL << (2 * sp->bits)) >= 9.2e18)
         return fail(c, FFA_ERR_INVALID, "fixed-point scale 2^%d too large for %lld particles", 2 * sp->bits,
@@ -250,6 +259,141 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
     ffa_sc_layout L;
     ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
     return hip_check(c, launch_sc_solve(c, p, sp, workspace, &L), "ffa_sc_solve");
+}
+
+// ---- stage-level SC entry points (FFAG_SC_new module functions) ----------
+namespace {
+int check_stage_ws(ffa_ctx_s *c, const void *ws, size_t ws_bytes, int64_t n, int32_t ns, int32_t nz, int32_t nx) {
+    if (!ws) return fail(c, FFA_ERR_INVALID, "workspace is NULL");
+    const size_t need = sc_stage_workspace_bytes(n, ns, nz, nx);
+    if (ws_bytes < need)
+        return fail(c, FFA_ERR_INVALID, "workspace of %zu bytes is smaller than the %zu this call needs", ws_bytes, need);
+    return FFA_OK;
+}
+int check_grid(ffa_ctx_s *c, int32_t ns, int32_t nz, int32_t nx) {
+    if (ns < 1 || ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [1, 4096]");
+    if (!fft_size_ok(nx) || !fft_size_ok(nz))
+        return fail(c, FFA_ERR_INVALID, "nx, nz must be in [32, 1024] with no prime factor above 5 (got %d, %d)",
+                    nx, nz);
+    return FFA_OK;
+}
+}  // namespace
+
+int ffa_sc_stage_workspace_size(int64_t n, int32_t ns, int32_t nz, int32_t nx, size_t *bytes) {
+    if (!bytes) return fail(nullptr, FFA_ERR_INVALID, "bytes is NULL");
+    if (n < 0 || n > 0xFFFFFFFFLL) return fail(nullptr, FFA_ERR_INVALID, "n must be in [0, 2^32)");
+    if (ns < 1 || ns > 4096 || nz < 1 || nx < 1 || nz > 1024 || nx > 1024)
+        return fail(nullptr, FFA_ERR_INVALID, "grid (%d, %d, %d) out of range", ns, nz, nx);
+    *bytes = sc_stage_workspace_bytes(n, ns, nz, nx);
+    return FFA_OK;
+}
+
+int ffa_sc_local_coords(void *ctx, const double *bunch, const uint8_t *survive, int64_t n, double speed_c, int32_t rk,
+                        double *out, double *count_sum, void *workspace, size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !count_sum) return fail(c, FFA_ERR_INVALID, "ctx or count_sum is NULL");
+    if (n < 0 || n > 0xFFFFFFFFLL) return fail(c, FFA_ERR_INVALID, "n must be in [0, 2^32)");
+    if (n > 0 && (!bunch || !survive || !out)) return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (!(speed_c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed_c must be positive");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, n, 1, 1, 1)) return rc;
+    return hip_check(c, launch_stage_local_coords(c, bunch, survive, n, speed_c, rk != 0, out, count_sum, workspace),
+                     "ffa_sc_local_coords");
+}
+
+int ffa_sc_divide_arc(void *ctx, const double *fi_mod, int64_t n, double *out, void *workspace, size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !out) return fail(c, FFA_ERR_INVALID, "ctx or out is NULL");
+    if (n < 0) return fail(c, FFA_ERR_INVALID, "negative count");
+    if (n > 0 && !fi_mod) return fail(c, FFA_ERR_INVALID, "fi_mod is NULL");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, 0, 1, 1, 1)) return rc;
+    return hip_check(c, launch_stage_divide_arc(c, fi_mod, n, out, workspace), "ffa_sc_divide_arc");
+}
+
+int ffa_sc_slices(void *ctx, const double *r, const double *z, const double *fi_mod, const double *r0, int64_t n,
+                  double slice_step, double fi_s, int32_t ns, double *x, double *r2d, double *z2d, double *x2d,
+                  int64_t stride, int32_t *counts, int32_t *slice_idx, double *sums, void *workspace,
+                  size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !counts || !sums) return fail(c, FFA_ERR_INVALID, "ctx, counts or sums is NULL");
+    if (n < 0 || n > 0xFFFFFFFFLL) return fail(c, FFA_ERR_INVALID, "n must be in [0, 2^32)");
+    if (ns < 1 || ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [1, 4096]");
+    if (stride < 0) return fail(c, FFA_ERR_INVALID, "negative stride");
+    if (n > 0 && (!r || !z || !fi_mod || !r0 || !x || !slice_idx || !z2d || !x2d))
+        return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (!(slice_step == slice_step)) return fail(c, FFA_ERR_INVALID, "slice_step is NaN");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, n, ns, 1, 1)) return rc;
+    return hip_check(c, launch_stage_slices(c, r, z, fi_mod, r0, n, slice_step, fi_s, ns, x, r2d, z2d, x2d, stride,
+                                            counts, slice_idx, sums, workspace),
+                     "ffa_sc_slices");
+}
+
+int ffa_sc_deposit(void *ctx, const double *z2d, const double *x2d, int64_t stride, const int32_t *counts, int32_t ns,
+                   int32_t nz, int32_t nx, double q_macro, double bunch_length, double xmin, double xmax, double zmin,
+                   double zmax, int32_t bits, float *rho, void *workspace, size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !counts || !rho) return fail(c, FFA_ERR_INVALID, "ctx, counts or rho is NULL");
+    if (ns < 1 || ns > 4096 || nz < 2 || nx < 2 || nz > 1024 || nx > 1024)
+        return fail(c, FFA_ERR_INVALID, "grid (%d, %d, %d) out of range", ns, nz, nx);
+    if (bits < 1 || bits > 20) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 20]");
+    if (stride < 0 || (stride > 0 && (!z2d || !x2d))) return fail(c, FFA_ERR_INVALID, "slice arrays missing");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, 0, ns, nz, nx)) return rc;
+    return hip_check(c, launch_stage_deposit(c, z2d, x2d, stride, counts, ns, nz, nx, q_macro, bunch_length, xmin, xmax,
+                                             zmin, zmax, bits, rho, workspace),
+                     "ffa_sc_deposit");
+}
+
+int ffa_sc_green(void *ctx, int32_t nx, int32_t nz, double dx, double dz, float *kernel, void *workspace,
+                 size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !kernel) return fail(c, FFA_ERR_INVALID, "ctx or kernel is NULL");
+    if (nx < 1 || nz < 1 || nx > 1024 || nz > 1024) return fail(c, FFA_ERR_INVALID, "nx, nz must be in [1, 1024]");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, 0, 2, nz, nx)) return rc;
+    return hip_check(c, launch_stage_green(c, nx, nz, dx, dz, kernel, workspace), "ffa_sc_green");
+}
+
+int ffa_sc_poisson(void *ctx, const float *rho, int32_t ns, int32_t nz, int32_t nx, double dx, double dz, float *phi,
+                   void *workspace, size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !rho || !phi) return fail(c, FFA_ERR_INVALID, "ctx, rho or phi is NULL");
+    if (int rc = check_grid(c, ns, nz, nx)) return rc;
+    if (!(dx > 0.0) || !(dz > 0.0)) return fail(c, FFA_ERR_INVALID, "dx and dz must be positive");
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, 0, ns, nz, nx)) return rc;
+    return hip_check(c, launch_stage_poisson(c, rho, ns, nz, nx, dx, dz, phi, workspace), "ffa_sc_poisson");
+}
+
+int ffa_sc_field(void *ctx, const float *phi, int32_t ns, int32_t nz, int32_t nx, double dr, double dz,
+                 double bunch_length, double mean_gamma_m1, float *ex, float *ez, void *workspace, size_t ws_bytes) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !phi || !ex || !ez) return fail(c, FFA_ERR_INVALID, "ctx, phi, ex or ez is NULL");
+    if (ns < 1 || ns > 4096 || nz < 2 || nx < 2 || nz > 1024 || nx > 1024)
+        return fail(c, FFA_ERR_INVALID, "grid (%d, %d, %d) out of range", ns, nz, nx);
+    if (int rc = check_stage_ws(c, workspace, ws_bytes, 0, 1, 1, 1)) return rc;
+    return hip_check(c, launch_stage_field(c, phi, ns, nz, nx, dr, dz, bunch_length, mean_gamma_m1, ex, ez, workspace),
+                     "ffa_sc_field");
+}
+
+int ffa_sc_total_field_1d(void *ctx, const double *s, const double *q, int32_t n, double *E) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(c, FFA_ERR_INVALID, "ctx is NULL");
+    if (n < 0) return fail(c, FFA_ERR_INVALID, "negative count");
+    if (n > 0 && (!s || !q || !E)) return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    return hip_check(c, launch_stage_total_field_1d(c, s, q, n, E), "ffa_sc_total_field_1d");
+}
+
+int ffa_sc_interpolate(void *ctx, const float *ex, const float *ez, int32_t ns, int32_t nz, int32_t nx, const double *x,
+                       const double *z, const double *pr0, const int32_t *slice, const int64_t *rows, int64_t n,
+                       double x0, double z0, double dx, double dz, double bunch_length, double *ex_p, double *ez_p,
+                       double *ef_p, double *bunch, int64_t n_rows) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c) return fail(c, FFA_ERR_INVALID, "ctx is NULL");
+    if (ns < 1 || nz < 2 || nx < 2) return fail(c, FFA_ERR_INVALID, "grid (%d, %d, %d) out of range", ns, nz, nx);
+    if (n < 0 || n_rows < 0) return fail(c, FFA_ERR_INVALID, "negative count");
+    if (n > 0 && (!ex || !ez || !x || !z || !pr0 || !slice || !ex_p || !ez_p || !ef_p))
+        return fail(c, FFA_ERR_INVALID, "an array pointer is NULL");
+    if (bunch && !rows) return fail(c, FFA_ERR_INVALID, "bunch given without rows");
+    return hip_check(c, launch_stage_interp(c, ex, ez, ns, nz, nx, x, z, pr0, slice, rows, n, x0, z0, dx, dz,
+                                            bunch_length, ex_p, ez_p, ef_p, bunch, n_rows),
+                     "ffa_sc_interpolate");
 }
 
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {

@@ -10,9 +10,11 @@
 // The state is converted to Cartesian once per launch and back once at the
 // end, instead of once per step as the reference does
 // (src/FFAG_MathTools.py:1969-2060): cos(phi), sin(phi) of the field rotation
-// are x/r, y/r, and the unwrapped azimuth is rebuilt from atan2 plus a count
-// of crossings of the atan2 branch cut, which is the reference's
-// |d phi| > pi unwrapping rule applied step by step.  Between steps the
+// are x/r, y/r.  The azimuth is carried unwrapped in FP64, advanced by the
+// exact turning angle of each step's chord (the reference's |d phi| < pi
+// unwrapping of atan2 differences, src/FFAG_MathTools.py:2035-2060), and the
+// stored azimuth at the end is the FP64 atan2 of the final position in the
+// turn the carried azimuth is in.  Between steps the
 // kernel carries u = gamma v, so the Boris step needs two reciprocal square
 // roots and one division (the reference recomputes gamma from v each step;
 // the two agree to FP64 rounding).
@@ -30,30 +32,6 @@ __device__ __forceinline__ float hw_log2f(float x) { return __builtin_amdgcn_log
 __device__ __forceinline__ float hw_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
 
 typedef const __attribute__((address_space(1))) float gfloat;
-
-// atan2 for finite positions away from the origin: t = min/max in [0, 1],
-// atan t = t P(t^2) with P of degree 7 fitted here (weighted least squares on
-// Chebyshev nodes, max error 4e-8 in FP64; evaluated in FP32 2e-7 rad for
-// |angle| < pi/2 and 3.4e-7 rad near +-pi, where the result's spacing is 2.4e-7),
-// then the octant and quadrant reflections.  Same branch cut as atan2.
-__device__ __forceinline__ float atan2_f32(float y, float x) {
-    const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    const float t = mn * rcpf(mx);
-    const float s = t * t;
-    float p = -0.004054246470332146f;
-    p = p * s + 0.021861769258975983f;
-    p = p * s - 0.05591056868433952f;
-    p = p * s + 0.09642065316438675f;
-    p = p * s - 0.1390857696533203f;
-    p = p * s + 0.1994655430316925f;
-    p = p * s - 0.33329859375953674f;
-    p = p * s + 0.9999993443489075f;
-    float r = t * p;
-    r = ay > ax ? 1.5707963267948966f - r : r;
-    r = x < 0.0f ? 3.141592653589793f - r : r;
-    return copysignf(r, y);
-}
 
 // Spiral RF gaps in FP32 (the FP64 path follows the reference's operation
 // order): first gap whose window, moved by the nearest whole number of
@@ -89,12 +67,14 @@ __device__ __forceinline__ void spiral_gap_f32(const ffa_fconst *__restrict__ k,
     gap = hit;
 }
 
-__device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float z,
+// Field at (r, th, z): th is the azimuth within a turn of the branch cut
+// (harmonics, gap direction, spiral-gap window), fim the same azimuth reduced
+// to the sector [0, P) in FP64 before rounding (the map index and the
+// order-0 field: fast_mod of the reference, src/FFAG_MathTools.py:1553-1557).
+__device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float fim, float z,
                                            float xf, float yf, float v0, double rf, float &Bz, float &Br,
                                            float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
                                            bool taylor = true) {
-    // fast_mod on the branch-cut azimuth (equal to the unwrapped one mod the sector)
-    const float fim = th - k->P * floorf(th * k->invP);
     float spiral = 0.0f, spiral_p = 0.0f;
     for (int a = 0; a < k->n_sp; ++a) spiral = spiral * r + k->sp[a];
     for (int a = 0; a < k->n_spp; ++a) spiral_p = spiral_p * r + k->spp[a];
@@ -265,8 +245,12 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     const double g0 = 1.0 / sqrt(1.0 - (vx0 * vx0 + vy0 * vy0 + vz0 * vz0) * inv_c2);
     double ux = g0 * vx0, uy = g0 * vy0, uz = g0 * vz0;
     double vx = vx0, vy = vy0, vz = vz0;
+    // azimuth of the position, carried unwrapped in FP64 from the launch's
+    // exact atan2 and advanced per step by the exact angle of the step's
+    // chord; the sector-local azimuth of the field is reduced from it in FP64
     const double th0 = atan2(y, x);
-    int wind = 0;
+    double thd = th0;
+    const double Pd = FFA_TWO_PI / (double)m.nsectors, invPd = (double)m.nsectors / FFA_TWO_PI;
     bool moved = false;
     const double q_over_m = q / mm;
     const bool aperture = m.aperture_enable != 0;
@@ -278,11 +262,13 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
         bool use_sur = false;
         float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
         const float xf = (float)x, yf = (float)y, zf = (float)z;
-        const float rr = hw_sqrtf(xf * xf + yf * yf);
-        const float th = atan2_f32(yf, xf);
+        const float r2f = xf * xf + yf * yf;
+        const float rr = hw_sqrtf(r2f);
+        const float th = (float)thd;
+        const float fim = (float)(thd - Pd * floor(thd * invPd));
         if constexpr (SUR) {
             float a, u, c;
-            mlp::inputs(md, rr, th, zf, a, u, c);
+            mlp::inputs(md, rr, fim, zf, a, u, c);
             use_sur = live && fabsf(c) <= 1.0f;
             if (__ballot(use_sur) != 0) {          // wave-uniform: all 64 lanes evaluate
                 float o[3];
@@ -300,7 +286,7 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
             esr = e0r + sc * d_r; esz = e0z + sc * d_z; esf = e0f + sc * d_f;
         }
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
-        fields_f32(kc, rr, th, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
+        fields_f32(kc, rr, th, fim, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
                    SUR ? (!use_sur && md->fallback != 0) : true);
         if constexpr (SUR) {
             Bz += sBz;
@@ -336,12 +322,16 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
         uz = mz + sc2 * (pxp * ty - pyp * tx) + eps * Ezc;
         const double ign = rsqrt(1.0 + (ux * ux + uy * uy + uz * uz) * inv_c2);
         vx = ux * ign; vy = uy * ign; vz = uz * ign;
-        const double xo = x, yo = y;
         x += vx * dt; y += vy * dt; z += vz * dt;
-        // atan2 branch cut (negative x axis) crossings
-        if (xo + x < 0.0) {
-            if (yo >= 0.0 && y < 0.0) ++wind;
-            else if (yo < 0.0 && y >= 0.0) --wind;
+        // azimuth advance: the chord from r to r + v dt turns by dth with
+        // tan dth = (r x v) dt / (r^2 + (r . v) dt); |dth| ~ 1e-3, so FP32 and
+        // atan t = t - t^3 / 3 leave ~1e-10 rad per step
+        {
+            const float vxf = (float)vx, vyf = (float)vy, dtf = (float)dt;
+            const float cr = (xf * vyf - yf * vxf) * dtf;
+            const float den = fmaf(xf * vxf + yf * vyf, dtf, r2f);
+            const float tq = cr * rcpf(den);
+            thd += (double)(tq - tq * tq * tq * (1.0f / 3.0f));
         }
         rf += s.dRF;
     }
@@ -351,7 +341,9 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     if (!moved) return;
     const double rn = sqrt(x * x + y * y);
     const double irn = 1.0 / rn;
-    const double fin = fi0 + (atan2(y, x) - th0 + FFA_TWO_PI * (double)wind);
+    // unwrapped azimuth: the exact end angle, in the turn the carried one is in
+    const double a_end = atan2(y, x);
+    const double fin = fi0 + (a_end - th0 + FFA_TWO_PI * rint((thd - a_end) * (1.0 / FFA_TWO_PI)));
     const double cn = x * irn, sn = y * irn;
     p.r[i] = rn;
     p.vr[i] = vx * cn + vy * sn;
@@ -362,19 +354,25 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
 }
 
 // ffa_bfield_eval mode 3: the static field exactly as the FP32 tier's push
-// evaluates it (fields_f32 on the FP32 map), widened to double.  The azimuth
-// is reduced to (-pi, pi] in FP64 first, the range the push's atan2 gives.
-__global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__restrict__ kc,
+// evaluates it at a position, widened to double: the FP64 Cartesian
+// position rounded to FP32 for the radius, the FP64 azimuth reduced to the
+// sector before rounding, fields_f32 on the FP32 map.
+__global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__restrict__ kc, int nsectors,
                                                            const double *__restrict__ r, const double *__restrict__ fi,
                                                            const double *__restrict__ z, int64_t n,
                                                            double *__restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
     if (i >= n) return;
-    double f = fi[i] - FFA_TWO_PI * floor(fi[i] * (1.0 / FFA_TWO_PI));
-    if (f > FFA_PI) f -= FFA_TWO_PI;
-    const float rr = (float)r[i], th = (float)f, zf = (float)z[i];
+    double s, c;
+    sincos(fi[i], &s, &c);
+    const double x = r[i] * c, y = r[i] * s;
+    const float xf = (float)x, yf = (float)y, zf = (float)z[i];
+    const float rr = hw_sqrtf(xf * xf + yf * yf);
+    const double thd = atan2(y, x);
+    const double Pd = FFA_TWO_PI / (double)nsectors;
+    const float th = (float)thd, fim = (float)(thd - Pd * floor(thd * ((double)nsectors / FFA_TWO_PI)));
     float Bz, Br, Bfi, Er, Efi, Ez, srf;
-    fields_f32(kc, rr, th, zf, rr * __cosf(th), rr * __sinf(th), 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
+    fields_f32(kc, rr, th, fim, zf, xf, yf, 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
     out[i] = (double)Bz;
     out[n + i] = (double)Bfi;
     out[2 * n + i] = (double)Br;
@@ -386,7 +384,7 @@ hipError_t launch_bfield_eval_f32(ffa_ctx_s *c, const double *r, const double *f
                                   double *out) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_bfield_eval_f32, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
-                       ctx_fconst_dev(c), r, fi, z, n, out);
+                       ctx_fconst_dev(c), c->host_machine.nsectors, r, fi, z, n, out);
     return hipGetLastError();
 }
 

@@ -113,11 +113,34 @@ hipError_t launch_spiral_gap(ffa_ctx_s *c, const ffa_spiral_gap *g);
 size_t sort_workspace_bytes(int64_t count);
 hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count, int nbins, void *ws,
                              const uint32_t **order_out);
+uint16_t *sort_key_buffer(void *ws);
+hipError_t launch_sort_keys16(ffa_ctx_s *c, int64_t count, int max_key, void *ws, const uint32_t **order_out,
+                              const uint16_t **keys_out);
 hipError_t launch_gather_particles(ffa_ctx_s *c, const ffa_particles *src, const ffa_particles *dst,
                                    const uint32_t *order, int64_t count, const int64_t *pid_src, int64_t *pid_dst);
 hipError_t mlp_upload_desc(ffa_ctx_s *c, const ffa_mlp_desc *d);   // -> FFA_CTX_MLP_OFF (synchronous)
 namespace mlp {
 size_t workspace_layout(int64_t n, ffa_mlp_layout &L);
 }
+size_t sc_stage_workspace_bytes(int64_t n, int32_t ns, int32_t nz, int32_t nx);
+hipError_t launch_stage_local_coords(ffa_ctx_s *c, const double *bunch, const uint8_t *survive, int64_t n,
+                                     double speed_c, int rk, double *out, double *count_sum, void *ws);
+hipError_t launch_stage_divide_arc(ffa_ctx_s *c, const double *fi, int64_t n, double *out, void *ws);
+hipError_t launch_stage_slices(ffa_ctx_s *c, const double *r, const double *z, const double *fim, const double *r0,
+                               int64_t n, double step, double fs, int ns, double *x, double *r2d, double *z2d,
+                               double *x2d, int64_t stride, int32_t *counts, int32_t *sl, double *sums, void *ws);
+hipError_t launch_stage_deposit(ffa_ctx_s *c, const double *z2d, const double *x2d, int64_t stride,
+                                const int32_t *counts, int ns, int nz, int nx, double q_macro, double bl, double xmin,
+                                double xmax, double zmin, double zmax, int bits, float *rho, void *ws);
+hipError_t launch_stage_green(ffa_ctx_s *c, int nx, int nz, double dx, double dz, float *out, void *ws);
+hipError_t launch_stage_poisson(ffa_ctx_s *c, const float *rho, int ns, int nz, int nx, double dx, double dz, float *phi,
+                                void *ws);
+hipError_t launch_stage_field(ffa_ctx_s *c, const float *phi, int ns, int nz, int nx, double dr, double dz, double bl,
+                              double mgm1, float *ex, float *ez, void *ws);
+hipError_t launch_stage_total_field_1d(ffa_ctx_s *c, const double *s, const double *q, int n, double *E);
+hipError_t launch_stage_interp(ffa_ctx_s *c, const float *ex, const float *ez, int ns, int nz, int nx, const double *x,
+                               const double *z, const double *pr0, const int32_t *slice, const int64_t *rows, int64_t n,
+                               double x0, double z0, double dx, double dz, double bl, double *ex_p, double *ez_p,
+                               double *ef_p, double *bunch, int64_t n_rows);
 hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp,
                            void *ws, const ffa_sc_layout *L);

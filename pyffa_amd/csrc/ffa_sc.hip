@@ -561,10 +561,10 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_slices(ffa_sc_params sp, c
                                                            unsigned long long *grid, int32_t *cell, int bz) {
     extern __shared__ unsigned long long lds[];
     const int nx = sp.nx, nz = sp.nz;
-    const int nbands = nz / bz;
+    const int nbands = (nz + bz - 1) / bz;                  // the last band may be shorter
     const int s = blockIdx.x / nbands, band = blockIdx.x - s * nbands;
-    const int z0 = band * bz, z1 = z0 + bz;
-    const int ncell = bz * nx;
+    const int z0 = band * bz, z1 = min(z0 + bz, nz);
+    const int ncell = (z1 - z0) * nx;
     const bool counts = sp.cell_counts != 0;
     unsigned long long *acc = lds;
     int32_t *cnt = reinterpret_cast<int32_t *>(lds + ncell);
@@ -615,10 +615,10 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_pairs(ffa_sc_params sp, co
                                                            int bz) {
     extern __shared__ unsigned long long lds[];
     const int nx = sp.nx, nz = sp.nz;
-    const int nbands = nz / bz;
+    const int nbands = (nz + bz - 1) / bz;                  // the last band may be shorter
     const int pl = blockIdx.x / nbands, band = blockIdx.x - pl * nbands;
-    const int z0 = band * bz, z1 = z0 + bz;
-    const int ncell = bz * nx;
+    const int z0 = band * bz, z1 = min(z0 + bz, nz);
+    const int ncell = (z1 - z0) * nx;
     const bool counts = sp.cell_counts != 0;
     const int npair = 2 * pl + 1 < sp.ns ? 2 : 1;          // an odd ns leaves the last plane one slice
     unsigned long long *acc = lds;
@@ -671,178 +671,251 @@ __global__ __launch_bounds__(BLOCK) void k_sc_deposit_pairs(ffa_sc_params sp, co
 }
 
 // ---------------------------------------------------------------------------
-// LDS Stockham FFT (radix 2, autosort).  A block transforms `lines` lines of
-// length N (N a power of two, N * lines == 1024) held in LDS.
+// LDS Stockham FFT (mixed radix 4, 2, 3, 5; autosort).  A block transforms
+// the lines of one tile, each of length N = 2^a 3^b 5^c <= 1024, held in LDS.
+// Power-of-two sizes take radix-4 stages and at most one radix-2 stage with
+// shift/mask index arithmetic (P2 = true); other sizes add radix-3 and
+// radix-5 stages and divide.
 // ---------------------------------------------------------------------------
 constexpr int FFT_PTS = 1024;
+constexpr int FFT_MAX_STAGES = 12;
+
+struct fft_plan {
+    int n, nst;
+    int radix[FFT_MAX_STAGES];
+};
+
+// Radix sequence of n (4s first, then one 2, then 3s, then 5s); nst = 0 when
+// n has a prime factor above 5.
+inline fft_plan make_fft_plan(int n) {
+    fft_plan p{};
+    p.n = n;
+    int m = n, k = 0;
+    while (m % 4 == 0 && k < FFT_MAX_STAGES) { p.radix[k++] = 4; m /= 4; }
+    if (m % 2 == 0 && k < FFT_MAX_STAGES) { p.radix[k++] = 2; m /= 2; }
+    while (m % 3 == 0 && k < FFT_MAX_STAGES) { p.radix[k++] = 3; m /= 3; }
+    while (m % 5 == 0 && k < FFT_MAX_STAGES) { p.radix[k++] = 5; m /= 5; }
+    p.nst = m == 1 ? k : 0;
+    return p;
+}
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// -i v (forward) or +i v (inverse)
+__device__ __forceinline__ float2 rot90(float2 v, bool inv) { return inv ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x); }
 
 __device__ __forceinline__ void load_twiddles(float2 *tw, const float2 *tab, int N) {
     for (int k = threadIdx.x; k < N; k += BLOCK) tw[k] = tab[k];
 }
 
-// In-LDS Stockham FFT of pts / N lines of N points: radix-4 stages
-//   a_j = x[q + s (p + j m)], j < 4, m = N / (4 s)
-//   y[q + s (4 p + k)] = w^(k p s) sum_j a_j (-+i)^(j k)
-// then one radix-2 stage when log2 N is odd; returns the buffer holding the
-// result.  Lines are ls >= N elements apart (padding against bank conflicts).
-__device__ float2 *lds_fft4(float2 *x, float2 *y, const float2 *tw, int N, int logN, bool inv, int pts = FFT_PTS,
-                            int ls = 0) {
+// q = a / d, r = a - q d (d a power of two with log2 ld when P2)
+template <bool P2>
+__device__ __forceinline__ int idiv(int a, int d, int ld) { return P2 ? (a >> ld) : a / d; }
+
+// In-LDS Stockham FFT of pts / N lines of N points.  Stage with radix R and
+// stride s (the product of the earlier radices), m = N / (R s), p < m, q < s:
+//   a_j = x[q + s (p + j m)], j < R
+//   y[q + s (R p + k)] = w^(k p s) sum_j a_j omega_R^(j k),  w = e^(-+2 pi i / N)
+// Every index stays below N.  Lines are ls >= N elements apart (padding
+// against bank conflicts).  Returns the buffer holding the result.
+template <bool P2>
+__device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, const fft_plan &pl, bool inv, int pts = FFT_PTS,
+                           int ls = 0) {
+    const int N = pl.n;
     if (ls == 0) ls = N;
-    const int qn = N >> 2;
+    const int lines = pts / N;
     int s = 1, logs = 0;
-    for (; 4 * s <= N; s <<= 2, logs += 2) {
-        const int m = N / (4 * s);
-        for (int t = threadIdx.x; t < (pts >> 2); t += BLOCK) {
-            const int line = t / qn, b = t - line * qn;
-            const int pp = b >> logs, q = b & (s - 1);
-            const float2 *xl = x + line * ls;
-            float2 *yl = y + line * ls;
-            const float2 a0 = xl[q + s * pp], a1 = xl[q + s * (pp + m)];
-            const float2 a2 = xl[q + s * (pp + 2 * m)], a3 = xl[q + s * (pp + 3 * m)];
-            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y), t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
-            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
-            const float2 d = make_float2(a1.x - a3.x, a1.y - a3.y);
-            const float2 t3 = inv ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);     // (+-i) d
+    for (int st = 0; st < pl.nst; ++st) {
+        const int R = pl.radix[st];
+        const int m = N / (R * s);
+        const int per = N / R;                           // butterflies per line
+        const int logper = P2 ? __builtin_ctz((unsigned)per) : 0;
+        for (int t = threadIdx.x; t < lines * per; t += BLOCK) {
+            const int line = idiv<P2>(t, per, logper), b = t - line * per;
+            const int pp = idiv<P2>(b, s, logs), q = b - pp * s;
+            const float2 *xl = x + line * ls + q + s * pp;
+            float2 *yl = y + line * ls + q + s * (R * pp);
             const int e = pp * s;
-            float2 *o = yl + q + s * (4 * pp);
-            o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
-            o[s] = cmul(make_float2(t1.x + t3.x, t1.y + t3.y), tw[e]);
-            o[2 * s] = cmul(make_float2(t0.x - t2.x, t0.y - t2.y), tw[2 * e]);
-            o[3 * s] = cmul(make_float2(t1.x - t3.x, t1.y - t3.y), tw[3 * e]);
+            const int sm = s * m;
+            if (R == 4) {
+                const float2 a0 = xl[0], a1 = xl[sm], a2 = xl[2 * sm], a3 = xl[3 * sm];
+                const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
+                const float2 t3 = rot90(csub(a1, a3), inv);
+                yl[0] = cadd(t0, t2);
+                yl[s] = cmul(cadd(t1, t3), tw[e]);
+                yl[2 * s] = cmul(csub(t0, t2), tw[2 * e]);
+                yl[3 * s] = cmul(csub(t1, t3), tw[3 * e]);
+            } else if (R == 2) {
+                const float2 a0 = xl[0], a1 = xl[sm];
+                yl[0] = cadd(a0, a1);
+                yl[s] = cmul(csub(a0, a1), tw[e]);
+            } else if (R == 3) {
+                // omega = e^(-+2 pi i / 3) = -1/2 -+ i sqrt(3)/2
+                const float2 a0 = xl[0], a1 = xl[sm], a2 = xl[2 * sm];
+                const float2 sum = cadd(a1, a2);
+                const float2 d = rot90(cscale(csub(a1, a2), 0.8660254037844386f), inv);
+                const float2 c = csub(a0, cscale(sum, 0.5f));
+                yl[0] = cadd(a0, sum);
+                yl[s] = cmul(cadd(c, d), tw[e]);
+                yl[2 * s] = cmul(csub(c, d), tw[2 * e]);
+            } else {
+                // radix 5 with c1 = cos(2 pi/5), c2 = cos(4 pi/5), s1 = sin(2 pi/5), s2 = sin(4 pi/5)
+                const float c1 = 0.30901699437494745f, c2 = -0.8090169943749475f;
+                const float s1 = 0.9510565162951535f, s2 = 0.5877852522924731f;
+                const float2 a0 = xl[0], a1 = xl[sm], a2 = xl[2 * sm], a3 = xl[3 * sm], a4 = xl[4 * sm];
+                const float2 t1 = cadd(a1, a4), t2 = cadd(a2, a3), t3 = csub(a1, a4), t4 = csub(a2, a3);
+                const float2 m1 = cadd(a0, cadd(cscale(t1, c1), cscale(t2, c2)));
+                const float2 m2 = cadd(a0, cadd(cscale(t1, c2), cscale(t2, c1)));
+                const float2 n1 = rot90(cadd(cscale(t3, s1), cscale(t4, s2)), inv);
+                const float2 n2 = rot90(csub(cscale(t3, s2), cscale(t4, s1)), inv);
+                yl[0] = cadd(a0, cadd(t1, t2));
+                yl[s] = cmul(cadd(m1, n1), tw[e]);
+                yl[2 * s] = cmul(cadd(m2, n2), tw[2 * e]);
+                yl[3 * s] = cmul(csub(m2, n2), tw[3 * e]);
+                yl[4 * s] = cmul(csub(m1, n1), tw[4 * e]);
+            }
         }
         __syncthreads();
         float2 *tmp = x; x = y; y = tmp;
+        s *= R;
+        if (P2) logs += R == 4 ? 2 : 1;
     }
-    if (s < N) {                        // radix 2, m = N / (2 s)
-        const int m = N / (2 * s), half = N >> 1;
-        for (int t = threadIdx.x; t < (pts >> 1); t += BLOCK) {
-            const int line = t / half, b = t - line * half;
-            const int pp = b >> logs, q = b & (s - 1);
-            const float2 *xl = x + line * ls;
-            float2 *yl = y + line * ls;
-            const float2 a = xl[q + s * pp], bb = xl[q + s * (pp + m)];
-            yl[q + s * (2 * pp)] = make_float2(a.x + bb.x, a.y + bb.y);
-            yl[q + s * (2 * pp + 1)] = cmul(make_float2(a.x - bb.x, a.y - bb.y), tw[pp * s]);
-        }
-        __syncthreads();
-        float2 *tmp = x; x = y; y = tmp;
-    }
-    (void)logN;
     return x;
 }
 
+// Rows of the packed slice pairs: a tile holds `lines` = FFT_PTS / nx whole
+// rows (global row = plane * nz + j); rows past the last of the last tile
+// are zero-filled and not stored.
+__host__ __device__ inline int row_lines(int nx) { return FFT_PTS / nx; }
+
 // rows, grids of up to 2^23 cells: int64 grid -> float, two slices packed
 // as re/im, forward FFT along x; grid -> cbuf.
+template <bool P2>
 __global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd_grid(const int64_t *grid, float2 *cbuf, ffa_sc_params sp,
-                                                       const ffa_sc_result *res, const float2 *twt, int lognx,
-                                                       int lognz) {
+                                                             const ffa_sc_result *res, const float2 *twt, fft_plan pl,
+                                                             int lognx, int lognz, int64_t rows) {
     extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
     float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
-    const int lines = FFT_PTS / nx;
+    const int lines = row_lines(nx), pts = lines * nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
     const double us = res->bad_grid != 0.0 ? 0.0 : res->qscale / (double)((int64_t)1 << (2 * sp.bits));
     load_twiddles(tw, twt, nx);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
-        const int l = e >> lognx, i = e & (nx - 1);
+    for (int e = threadIdx.x; e < pts; e += BLOCK) {
+        const int l = idiv<P2>(e, nx, lognx), i = e - l * nx;
         const int64_t row = row0 + l;
-        const int64_t plane = row >> lognz, j = row & (nz - 1);
-        const int64_t sa = 2 * plane, sb = sa + 1;
-        const float re = (float)((double)grid[(sa * nz + j) * nx + i] * us);
-        const float im = sb < ns ? (float)((double)grid[(sb * nz + j) * nx + i] * us) : 0.0f;
-        b0[e] = make_float2(re, im);
+        float2 v = make_float2(0.0f, 0.0f);
+        if (row < rows) {
+            const int64_t plane = P2 ? (row >> lognz) : row / nz, j = row - plane * nz;
+            const int64_t sa = 2 * plane, sb = sa + 1;
+            v.x = (float)((double)grid[(sa * nz + j) * nx + i] * us);
+            v.y = sb < ns ? (float)((double)grid[(sb * nz + j) * nx + i] * us) : 0.0f;
+        }
+        b0[e] = v;
     }
     __syncthreads();
-    const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, false);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) cbuf[row0 * nx + e] = o[e];
+    const float2 *o = lds_fft<P2>(b0, b1, tw, pl, false, pts);
+    for (int e = threadIdx.x; e < pts; e += BLOCK)
+        if (row0 * nx + e < rows * nx) cbuf[row0 * nx + e] = o[e];
 }
 
 // rows, larger grids: forward FFT along x of the packed slice pairs, in
 // place in cbuf.
-__global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(float2 *cbuf, ffa_sc_params sp, const float2 *twt,
-                                                       int lognx) {
+template <bool P2>
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_fwd(float2 *cbuf, ffa_sc_params sp, const float2 *twt, fft_plan pl,
+                                                        int64_t rows) {
     extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
     float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
     const int nx = sp.nx;
-    const int lines = FFT_PTS / nx;
-    const int64_t row0 = (int64_t)blockIdx.x * lines;          // global row = plane*nz + j
+    const int lines = row_lines(nx), pts = lines * nx;
+    const int64_t row0 = (int64_t)blockIdx.x * lines;
+    const int64_t lim = rows * nx - row0 * nx;
     load_twiddles(tw, twt, nx);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
+    for (int e = threadIdx.x; e < pts; e += BLOCK) b0[e] = e < lim ? cbuf[row0 * nx + e] : make_float2(0.0f, 0.0f);
     __syncthreads();
-    const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, false);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) cbuf[row0 * nx + e] = o[e];
+    const float2 *o = lds_fft<P2>(b0, b1, tw, pl, false, pts);
+    for (int e = threadIdx.x; e < pts; e += BLOCK)
+        if (e < lim) cbuf[row0 * nx + e] = o[e];
 }
 
 // columns: forward FFT along z, multiply by the periodic Green's function
 // (make_fft_kernel_open_numba, src/FFAG_SC_new.py:731-760), inverse FFT.
-// Columns per workgroup: COL_PTS / nz (at most nx).  Wider tiles (4096
-// points) measured slower at nz = 256: fewer workgroups, lower occupancy.
+// Columns per workgroup: COL_PTS / nz (at most nx); the last group of a
+// plane may hold fewer.  Wider tiles (4096 points) measured slower at
+// nz = 256: fewer workgroups, lower occupancy.
 constexpr int COL_PTS = 1024;
 
 __host__ __device__ inline int col_width(int nx, int nz) { return COL_PTS / nz < nx ? COL_PTS / nz : nx; }
+__host__ __device__ inline int col_groups(int nx, int nz) { return (nx + col_width(nx, nz) - 1) / col_width(nx, nz); }
 
+template <bool P2>
 __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_params sp, const float2 *twt,
-                                                         const float *green, int lognz) {
+                                                          const float *green, fft_plan pl) {
     // dynamic LDS: b0, b1 (COL_PTS + 32 each, padded columns), tw[nz], twi[nz]
     extern __shared__ float2 lds2[];
     float2 *b0 = lds2, *b1 = lds2 + (COL_PTS + 32), *tw = lds2 + 2 * (COL_PTS + 32), *twi = tw + sp.nz;
     const int nx = sp.nx, nz = sp.nz;
     const int W = col_width(nx, nz);                  // columns per block
-    const int PTS = W * nz;
     const int LS = nz + 1;                            // padded column stride in LDS
-    const int cpp = nx / W;                           // column groups per plane
+    const int cpp = col_groups(nx, nz);               // column groups per plane
     // neighbouring column groups share cache lines: keep them on one XCD
     const unsigned blk = xcd_block();
     const int64_t plane = blk / cpp;
-    const int c0 = (blk - plane * cpp) * W;
+    const int c0 = (int)(blk - plane * cpp) * W;
+    const int Wb = min(W, nx - c0);                   // columns of this block
+    const int PTS = W * nz;
     float2 *base = cbuf + plane * (int64_t)nz * nx;
     load_twiddles(tw, twt + 2 * TW_STRIDE, nz);
     load_twiddles(twi, twt + 3 * TW_STRIDE, nz);
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
-        b0[c * LS + j] = base[(int64_t)j * nx + c0 + c];
+        b0[c * LS + j] = c < Wb ? base[(int64_t)j * nx + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    float2 *o = lds_fft4(b0, b1, tw, nz, lognz, false, PTS, LS);
+    float2 *o = lds_fft<P2>(b0, b1, tw, pl, false, PTS, LS);
     float2 *other = (o == b0) ? b1 : b0;
     // Green's function of the block's columns: one contiguous run of the
     // transposed table
     const float *gcol = green + (int64_t)c0 * nz;
-    for (int e = threadIdx.x; e < PTS; e += BLOCK) {
+    for (int e = threadIdx.x; e < Wb * nz; e += BLOCK) {
         const int c = e / nz, j = e - c * nz;
         const float g = gcol[e];
         float2 &v = o[c * LS + j];
         v = make_float2(v.x * g, v.y * g);
     }
     __syncthreads();
-    float2 *r = lds_fft4(o, other, twi, nz, lognz, true, PTS, LS);
+    float2 *r = lds_fft<P2>(o, other, twi, pl, true, PTS, LS);
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
         const int j = e / W, c = e - j * W;
-        base[(int64_t)j * nx + c0 + c] = r[c * LS + j];
+        if (c < Wb) base[(int64_t)j * nx + c0 + c] = r[c * LS + j];
     }
 }
 
 // rows: inverse FFT along x, normalise by 1/(nx nz), unpack the slice pair.
+template <bool P2>
 __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, float *phi, ffa_sc_params sp,
-                                                       const float2 *twt, int lognx, int lognz) {
+                                                        const float2 *twt, fft_plan pl, int lognx, int lognz,
+                                                        int64_t rows) {
     extern __shared__ float2 lds2[];           // b0[FFT_PTS], b1[FFT_PTS], tw[nx]
     float2 *b0 = lds2, *b1 = lds2 + FFT_PTS, *tw = lds2 + 2 * FFT_PTS;
     const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
-    const int lines = FFT_PTS / nx;
+    const int lines = row_lines(nx), pts = lines * nx;
     const int64_t row0 = (int64_t)blockIdx.x * lines;
+    const int64_t lim = rows * nx - row0 * nx;
     load_twiddles(tw, twt + TW_STRIDE, nx);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) b0[e] = cbuf[row0 * nx + e];
+    for (int e = threadIdx.x; e < pts; e += BLOCK) b0[e] = e < lim ? cbuf[row0 * nx + e] : make_float2(0.0f, 0.0f);
     __syncthreads();
-    const float2 *o = lds_fft4(b0, b1, tw, nx, lognx, true);
+    const float2 *o = lds_fft<P2>(b0, b1, tw, pl, true, pts);
     const float norm = 1.0f / (float)(nx * nz);
-    for (int e = threadIdx.x; e < FFT_PTS; e += BLOCK) {
-        const int l = e >> lognx, i = e & (nx - 1);
+    for (int e = threadIdx.x; e < pts; e += BLOCK) {
+        if (e >= lim) continue;
+        const int l = idiv<P2>(e, nx, lognx), i = e - l * nx;
         const int64_t row = row0 + l;
-        const int64_t plane = row >> lognz, j = row & (nz - 1);
+        const int64_t plane = P2 ? (row >> lognz) : row / nz, j = row - plane * nz;
         const int64_t sa = 2 * plane, sb = sa + 1;
         phi[(sa * nz + j) * nx + i] = o[e].x * norm;
         if (sb < ns) phi[(sb * nz + j) * nx + i] = o[e].y * norm;
@@ -880,24 +953,37 @@ __device__ __forceinline__ float2 cell_field(float pc, float pu, float pd, float
     return make_float2((float)ex, (float)ez);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *exz, ffa_sc_params sp,
-                                                   const ffa_sc_result *res, int lognx) {
+// PLANAR: (Ex, Ez) stored as two (ns, nz, nx) cubes ex, ez (the reference's
+// Ex_cube / Ez_cube) instead of the interleaved exz cube.
+template <bool P2, bool PLANAR>
+__global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *exz, float *ex, float *ez,
+                                                   ffa_sc_params sp, const ffa_sc_result *res, int lognx) {
     const int nx = sp.nx, nz = sp.nz;
-    const int nb = nz / FIELD_ROWS;
+    const int nb = (nz + FIELD_ROWS - 1) / FIELD_ROWS;
     const int64_t total = (int64_t)sp.ns * nb * nx;
     // the stencil rows j0 - 1 and j0 + FIELD_ROWS belong to neighbouring
     // bands: keep neighbours on one XCD so each row is fetched into one L2
     const int64_t t = xcd_block() * (int64_t)BLOCK + threadIdx.x;
     if (t >= total) return;
-    const int i = (int)(t & (nx - 1));
-    const int64_t r = t >> lognx;
+    const int64_t r = P2 ? (t >> lognx) : t / nx;
+    const int i = (int)(t - r * nx);
     const int band = (int)(r % nb);
     const int64_t k = r / nb;
     const int j0 = band * FIELD_ROWS;
+    const int rows = min(FIELD_ROWS, nz - j0);        // the last band may be shorter
     const int64_t base = k * (int64_t)nz * nx;
+    auto store = [&](int j, float2 v) {
+        const int64_t o = base + (int64_t)j * nx + i;
+        if constexpr (PLANAR) {
+            ex[o] = v.x;
+            ez[o] = v.y;
+        } else {
+            exz[o] = v;
+        }
+    };
     const double dr = res->dx, dz = res->dz, bl = res->bunch_length;
     if (res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0) {
-        for (int c = 0; c < FIELD_ROWS; ++c) exz[base + (int64_t)(j0 + c) * nx + i] = make_float2(0.0f, 0.0f);
+        for (int c = 0; c < rows; ++c) store(j0 + c, make_float2(0.0f, 0.0f));
         return;
     }
     const double gamma = res->mean_gamma_m1 + 1.0;
@@ -913,13 +999,13 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
         col[c] = P[(int64_t)j * nx + i];
     }
     for (int c = 0; c < FIELD_ROWS; ++c) {
-        lf[c] = P[(int64_t)(j0 + c) * nx + il];
-        rt[c] = P[(int64_t)(j0 + c) * nx + ir];
+        const int j = min(j0 + c, nz - 1);              // rows past the grid are loaded but not stored
+        lf[c] = P[(int64_t)j * nx + il];
+        rt[c] = P[(int64_t)j * nx + ir];
     }
-    for (int c = 0; c < FIELD_ROWS; ++c) {
+    for (int c = 0; c < rows; ++c) {
         const int j = j0 + c;
-        exz[base + (int64_t)j * nx + i] =
-            cell_field(col[c + 1], col[c + 2], col[c], lf[c], rt[c], i, j, nx, nz, dr, dz, i2r, i2z, ig2);
+        store(j, cell_field(col[c + 1], col[c + 2], col[c], lf[c], rt[c], i, j, nx, nz, dr, dz, i2r, i2z, ig2));
     }
 }
 
@@ -1027,6 +1113,62 @@ inline int ilog2(int v) {
     return l;
 }
 
+inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Periodic Poisson solve of every slice pair: forward row FFTs (reading and
+// scaling the int64 grid when `grid` is given, else the packed charge already
+// in cbuf), column FFT x Green's function x inverse column FFT, inverse row
+// FFTs unpacked to phi.  twt / green: the per-solve tables.
+void launch_fft_poisson(hipStream_t st, const ffa_sc_params &sp, const ffa_sc_result *res, const int64_t *grid,
+                        float2 *cbuf, float *phi, const float2 *twt, const float *green) {
+    const bool p2 = is_pow2(sp.nx) && is_pow2(sp.nz);
+    const fft_plan px = make_fft_plan(sp.nx), pz = make_fft_plan(sp.nz);
+    const int lognx = ilog2(sp.nx), lognz = ilog2(sp.nz);
+    const int64_t planes = (sp.ns + 1) / 2;
+    const int64_t rows = planes * sp.nz;
+    const int lines = row_lines(sp.nx);
+    const unsigned rblocks = (unsigned)((rows + lines - 1) / lines);
+    const unsigned cblocks = (unsigned)(planes * col_groups(sp.nx, sp.nz));
+    const size_t rows_lds = sizeof(float2) * (2 * FFT_PTS + sp.nx);
+    const size_t cols_lds = sizeof(float2) * (2 * (COL_PTS + 32) + 2 * sp.nz);
+    if (p2) {
+        if (grid)
+            hipLaunchKernelGGL(k_fft_rows_fwd_grid<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, grid, cbuf, sp, res,
+                               twt, px, lognx, lognz, rows);
+        else
+            hipLaunchKernelGGL(k_fft_rows_fwd<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, sp, twt, px, rows);
+        hipLaunchKernelGGL(k_fft_cols_green<true>, dim3(cblocks), dim3(BLOCK), cols_lds, st, cbuf, sp, twt, green, pz);
+        hipLaunchKernelGGL(k_fft_rows_inv<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
+                           lognx, lognz, rows);
+    } else {
+        if (grid)
+            hipLaunchKernelGGL(k_fft_rows_fwd_grid<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, grid, cbuf, sp,
+                               res, twt, px, lognx, lognz, rows);
+        else
+            hipLaunchKernelGGL(k_fft_rows_fwd<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, sp, twt, px,
+                               rows);
+        hipLaunchKernelGGL(k_fft_cols_green<false>, dim3(cblocks), dim3(BLOCK), cols_lds, st, cbuf, sp, twt, green, pz);
+        hipLaunchKernelGGL(k_fft_rows_inv<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
+                           lognx, lognz, rows);
+    }
+}
+
+// -grad(phi) / gamma^2 of every cell: interleaved exz, or the planar ex / ez
+// cubes when exz is null.
+void launch_field(hipStream_t st, const ffa_sc_params &sp, const ffa_sc_result *res, const float *phi, float2 *exz,
+                  float *ex, float *ez) {
+    const int64_t threads = (int64_t)sp.ns * ((sp.nz + FIELD_ROWS - 1) / FIELD_ROWS) * sp.nx;
+    const dim3 g(gridn(threads));
+    const int lognx = ilog2(sp.nx);
+    if (is_pow2(sp.nx)) {
+        if (exz) hipLaunchKernelGGL((k_sc_field<true, false>), g, dim3(BLOCK), 0, st, phi, exz, ex, ez, sp, res, lognx);
+        else hipLaunchKernelGGL((k_sc_field<true, true>), g, dim3(BLOCK), 0, st, phi, exz, ex, ez, sp, res, lognx);
+    } else {
+        if (exz) hipLaunchKernelGGL((k_sc_field<false, false>), g, dim3(BLOCK), 0, st, phi, exz, ex, ez, sp, res, lognx);
+        else hipLaunchKernelGGL((k_sc_field<false, true>), g, dim3(BLOCK), 0, st, phi, exz, ex, ez, sp, res, lognx);
+    }
+}
+
 }  // namespace
 
 int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *L) {
@@ -1096,35 +1238,19 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     float *green = reinterpret_cast<float *>(w + L->tables + 8 * 4 * (size_t)TW_STRIDE);
     hipLaunchKernelGGL(k_sc_rowsort, dim3((unsigned)sp->ns), dim3(ROW_BLOCK), 0, st, *sp, res, sstart, rec, rec2,
                        rstart, twt, green);
+    const int nbands = (sp->nz + bz - 1) / bz;
     if (pairs)
-        hipLaunchKernelGGL(k_sc_deposit_pairs, dim3((unsigned)(((sp->ns + 1) / 2) * (sp->nz / bz))), dim3(BLOCK),
-                           tile_lds, st, *sp, res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, cbuf,
-                           bz);
+        hipLaunchKernelGGL(k_sc_deposit_pairs, dim3((unsigned)(((sp->ns + 1) / 2) * nbands)), dim3(BLOCK), tile_lds,
+                           st, *sp, res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, cbuf, bz);
     else
-        hipLaunchKernelGGL(k_sc_deposit_slices, dim3((unsigned)(sp->ns * (sp->nz / bz))), dim3(BLOCK), tile_lds, st,
-                           *sp, res, rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
-    const int lognx = ilog2(sp->nx), lognz = ilog2(sp->nz);
-    const int64_t planes = (sp->ns + 1) / 2;
-    const int64_t rows = planes * sp->nz;
-    const size_t rows_lds = sizeof(float2) * (2 * FFT_PTS + sp->nx);
-    const size_t cols_lds = sizeof(float2) * (2 * (COL_PTS + 32) + 2 * sp->nz);
-    if (pairs)
-        hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf,
-                           *sp, twt, lognx);
-    else
-        hipLaunchKernelGGL(k_fft_rows_fwd_grid, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st,
-                           grid, cbuf, *sp, res, twt, lognx, lognz);
-    hipLaunchKernelGGL(k_fft_cols_green, dim3((unsigned)(planes * sp->nx / col_width(sp->nx, sp->nz))), dim3(BLOCK),
-                       cols_lds, st, cbuf, *sp, twt, green, lognz);
-    hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)(rows * sp->nx / FFT_PTS)), dim3(BLOCK), rows_lds, st, cbuf, phi,
-                       *sp, twt, lognx, lognz);
+        hipLaunchKernelGGL(k_sc_deposit_slices, dim3((unsigned)(sp->ns * nbands)), dim3(BLOCK), tile_lds, st, *sp, res,
+                           rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
+    launch_fft_poisson(st, *sp, res, pairs ? nullptr : grid, cbuf, phi, twt, green);
     // The gather from phi reads 12 potential values per particle instead of 4
     // field pairs: cheaper while the potential stays cache-resident (measured
     // faster at 4.2M cells, slower at 33.5M), so larger grids store the cubes
     const bool cubes = sp->field_cubes != 0 || C > FROM_PHI_MAX_CELLS;
-    if (cubes)
-        hipLaunchKernelGGL(k_sc_field, dim3(gridn((int64_t)(C / FIELD_ROWS))), dim3(BLOCK), 0, st, phi, exz, *sp, res,
-                           lognx);
+    if (cubes) launch_field(st, *sp, res, phi, exz, nullptr, nullptr);
     if (p->n > 0) {
         const dim3 g(gridn(p->n));
         if (p->esc_f32) {
@@ -1143,5 +1269,495 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
                                    ppr0, pslice);
         }
     }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Stage-level entry points: the module functions of FFAG_SC_new, each on its
+// own (survivor compaction, covering arc, slicing and moments, deposit,
+// Green's function, Poisson solve, field, longitudinal field, gather), for
+// callers that drive the stages themselves.  Same arithmetic as the fused
+// solve above where the reference's stage and the fused pipeline agree; the
+// deposit is the fixed-point one (integer sums, bit-reproducible).
+// ===========================================================================
+namespace {
+
+// GetLocalCoordinates_2p5(_rk) (src/FFAG_SC_new.py:74-148): key 0 for the
+// survivors, 1 for the others; a stable sort by key keeps the survivors in
+// row order first.
+__global__ __launch_bounds__(BLOCK) void k_stage_survive_keys(const uint8_t *survive, int64_t n, uint16_t *keys) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i < n) keys[i] = survive[i] ? 0 : 1;
+}
+
+__device__ __forceinline__ double row_gamma(const double *row, int rk, double c, double E0J) {
+    if (rk) return row[5] / E0J;
+    const double r = row[0], drdt = row[1], dzdt = row[3], rd = r * row[5];
+    const double v2 = drdt * drdt + dzdt * dzdt + rd * rd;
+    return 1.0 / sqrt(1.0 - v2 / (c * c));
+}
+
+// out planes (n each): r, fi, z, gamma, fi mod 2 pi, source row index
+__global__ __launch_bounds__(BLOCK) void k_stage_local_coords(const double *bunch, const uint8_t *survive,
+                                                             const uint32_t *order, int64_t n, double c, int rk,
+                                                             double E0J, double *out) {
+    const int64_t j = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const int64_t i = order[j];
+    if (!survive[i]) return;                      // survivors occupy positions [0, count)
+    const double *row = bunch + i * 16;
+    const double fi = row[4];
+    out[j] = row[0];
+    out[n + j] = fi;
+    out[2 * n + j] = row[2];
+    out[3 * n + j] = row_gamma(row, rk, c, E0J);
+    out[4 * n + j] = fi - FFA_TWO_PI * floor(fi / FFA_TWO_PI);
+    out[5 * n + j] = (double)i;
+}
+
+// per-block (count, sum gamma) over the survivors, fixed partition
+__global__ __launch_bounds__(BLOCK) void k_stage_gamma_partials(const double *bunch, const uint8_t *survive, int64_t n,
+                                                               double c, int rk, double E0J, double *part) {
+    __shared__ double sh[2][BLOCK / 64];
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)NB * BLOCK)
+        if (survive[i]) {
+            v[0] += 1.0;
+            v[1] += row_gamma(bunch + i * 16, rk, c, E0J);
+        }
+    const int op[2] = {0, 0};
+    block_reduce<2>(sh, v, op);
+    if (threadIdx.x == 0) { part[16 * blockIdx.x] = v[0]; part[16 * blockIdx.x + 1] = v[1]; }
+}
+
+// combine operation of each partial column: 0 sum, 1 min, 2 max
+struct reduce_ops {
+    int op[8];
+};
+
+// generic final: the NV partial columns of the fixed block partition
+template <int NV>
+__global__ __launch_bounds__(BLOCK) void k_stage_reduce_final(const double *part, reduce_ops ops, double *out) {
+    __shared__ double sh[NV][BLOCK / 64];
+    int op[NV];
+    double v[NV];
+    for (int k = 0; k < NV; ++k) {
+        op[k] = ops.op[k];
+        v[k] = op[k] == 0 ? 0.0 : (op[k] == 1 ? 1e30 : -1e30);
+    }
+    for (int b = threadIdx.x; b < NB; b += BLOCK)
+        for (int k = 0; k < NV; ++k) v[k] = combine(op[k], v[k], part[16 * b + k]);
+    block_reduce<NV>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) out[k] = v[k];
+}
+
+// divide_arc2 (src/FFAG_SC_new.py:534-566): per-block min / max of the
+// azimuths shifted by 0, pi, pi/2, 3 pi/2, reduced mod 2 pi
+__global__ __launch_bounds__(BLOCK) void k_stage_arc_partials(const double *fi, int64_t n, double *part) {
+    __shared__ double sh[8][BLOCK / 64];
+    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
+    double v[8] = {1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)NB * BLOCK)
+        for (int k = 0; k < 4; ++k) {
+            const double a = fi[i] + shifts[k], b = 2.0 * FFA_PI;
+            const double q = a - b * floor(a / b);
+            v[k] = q < v[k] ? q : v[k];
+            v[4 + k] = q > v[4 + k] ? q : v[4 + k];
+        }
+    const int op[8] = {1, 1, 1, 1, 2, 2, 2, 2};
+    block_reduce<8>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) part[16 * blockIdx.x + k] = v[k];
+}
+
+__global__ void k_stage_arc_final(const double *mm, double *out) {
+    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
+    int best = 0;
+    double bs = mm[4] - mm[0];
+    for (int k = 1; k < 4; ++k)
+        if (mm[4 + k] - mm[k] < bs) { bs = mm[4 + k] - mm[k]; best = k; }
+    out[0] = mm[best] - shifts[best];
+    out[1] = mm[4 + best] - shifts[best];
+    out[2] = bs;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_stage_zero_i32(int32_t *a, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_stage_zero_u64(unsigned long long *a, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) a[i] = 0ull;
+}
+
+// ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531):
+// x = r - r0, the slice of each particle, the per-slice counts, per-block
+// moment partials; key = slice for the stable placement that follows
+__global__ __launch_bounds__(BLOCK) void k_stage_slice_keys(const double *r, const double *z, const double *fim,
+                                                           const double *r0, int64_t n, double step, double fs, int ns,
+                                                           double *x_out, int32_t *sl_out, uint16_t *keys,
+                                                           int32_t *counts, double *part) {
+    __shared__ double sh[4][BLOCK / 64];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)NB * BLOCK) {
+        const double x = r[i] - r0[i], zz = z[i];
+        x_out[i] = x;
+        v[0] += x; v[1] += zz; v[2] += x * x; v[3] += zz * zz;
+        double f = fim[i];
+        if (f < fs) f += FFA_TWO_PI;
+        double q = (f - fs) / step;
+        q = q > -1e9 ? (q < 1e9 ? q : 1e9) : -1e9;
+        int64_t s = (int64_t)q;
+        if (s < 0) s = 0; else if (s >= ns) s = ns - 1;
+        sl_out[i] = (int32_t)s;
+        keys[i] = (uint16_t)s;
+        atomicAdd(&counts[s], 1);
+    }
+    const int op[4] = {0, 0, 0, 0};
+    block_reduce<4>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[16 * blockIdx.x + k] = v[k];
+}
+
+// exclusive scan of ns <= 4096 counts in one block
+__global__ __launch_bounds__(BLOCK) void k_stage_scan_counts(const int32_t *counts, int ns, int32_t *start) {
+    __shared__ int32_t tot[BLOCK];
+    const int per = (ns + BLOCK - 1) / BLOCK;
+    const int k0 = threadIdx.x * per, k1 = min(k0 + per, ns);
+    int32_t own = 0;
+    for (int k = k0; k < k1; ++k) own += counts[k];
+    tot[threadIdx.x] = own;
+    __syncthreads();
+    for (int w = 1; w < BLOCK; w <<= 1) {
+        const int32_t add = threadIdx.x >= w ? tot[threadIdx.x - w] : 0;
+        __syncthreads();
+        tot[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int32_t acc = tot[threadIdx.x] - own;
+    for (int k = k0; k < k1; ++k) { start[k] = acc; acc += counts[k]; }
+}
+
+// the per-slice lists in particle order: sorted position j holds particle
+// order[j] of slice key[j] at place j - start[key[j]] (places at or beyond
+// the row stride are not written)
+__global__ __launch_bounds__(BLOCK) void k_stage_slice_place(const uint32_t *order, const uint16_t *key,
+                                                            const int32_t *start, const double *r, const double *z,
+                                                            const double *x, int64_t n, int64_t stride, double *r2d,
+                                                            double *z2d, double *x2d) {
+    const int64_t j = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const int64_t i = order[j], s = key[j];
+    const int64_t pos = j - start[s];
+    if (pos >= stride) return;
+    const int64_t o = s * stride + pos;
+    if (r2d) r2d[o] = r[i];
+    z2d[o] = z[i];
+    x2d[o] = x[i];
+}
+
+// DistributeChargeSBS_Njit (src/FFAG_SC_new.py:244-312) in fixed point: the
+// CIC cell and clamped fractions of the reference, rounded to `bits`
+// fractional bits, added as integers (one workgroup per slice)
+__global__ __launch_bounds__(BLOCK) void k_stage_deposit(const double *z2d, const double *x2d, int64_t stride,
+                                                        const int32_t *counts, int nz, int nx, double xmin, double zmin,
+                                                        double inv_dx, double inv_dz, int bits,
+                                                        unsigned long long *grid) {
+    const int s = blockIdx.x;
+    const int64_t cnt = min((int64_t)counts[s], stride);
+    const uint64_t one = (uint64_t)1 << bits;
+    const double scale = (double)one;
+    unsigned long long *g = grid + (size_t)s * nz * nx;
+    for (int64_t k = threadIdx.x; k < cnt; k += BLOCK) {
+        const double rx = (x2d[s * stride + k] - xmin) * inv_dx, rz = (z2d[s * stride + k] - zmin) * inv_dz;
+        const double crx = rx > -1e9 ? (rx < 1e9 ? rx : 1e9) : -1e9;
+        const double crz = rz > -1e9 ? (rz < 1e9 ? rz : 1e9) : -1e9;
+        int64_t ix = (int64_t)floor(crx), iz = (int64_t)floor(crz);
+        double fx = rx - (double)ix, fz = rz - (double)iz;
+        if (ix < 0) { ix = 0; fx = 0.0; } else if (ix >= nx - 1) { ix = nx - 2; fx = 1.0; }
+        if (iz < 0) { iz = 0; fz = 0.0; } else if (iz >= nz - 1) { iz = nz - 2; fz = 1.0; }
+        const uint64_t a = (uint64_t)rint(fx * scale), b = (uint64_t)rint(fz * scale);
+        const size_t o = (size_t)iz * nx + ix;
+        atomicAdd(&g[o], (unsigned long long)((one - a) * (one - b)));
+        atomicAdd(&g[o + 1], (unsigned long long)(a * (one - b)));
+        atomicAdd(&g[o + nx], (unsigned long long)((one - a) * b));
+        atomicAdd(&g[o + nx + 1], (unsigned long long)(a * b));
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_stage_grid_to_rho(const int64_t *grid, int64_t C, double us, float *rho) {
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < C; i += (int64_t)gridDim.x * BLOCK)
+        rho[i] = (float)((double)grid[i] * us);
+}
+
+// the scalars the FFT tables and the field kernel read
+__global__ void k_stage_set_result(ffa_sc_result *res, double dx, double dz, double bl, double mgm1, double bad) {
+    res->dx = dx;
+    res->dz = dz;
+    res->bunch_length = bl;
+    res->mean_gamma_m1 = mgm1;
+    res->bad_grid = bad;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_stage_tables(ffa_sc_params sp, const ffa_sc_result *res, float2 *tw,
+                                                       float *green) {
+    const int64_t n_tab = 4 * TW_STRIDE + (int64_t)sp.nx * sp.nz;
+    for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < n_tab; t += (int64_t)gridDim.x * BLOCK)
+        sc_table_entry(t, tw, green, sp, res);
+}
+
+// make_fft_kernel_open_numba's (nz, nx) layout from the transposed table
+__global__ __launch_bounds__(BLOCK) void k_stage_green_out(const float *green_t, int nz, int nx, float *out) {
+    const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (t >= (int64_t)nz * nx) return;
+    const int j = (int)(t / nx), i = (int)(t - (int64_t)j * nx);
+    out[t] = green_t[(int64_t)i * nz + j];
+}
+
+// real slices -> packed pairs (re: slice 2p, im: slice 2p + 1)
+__global__ __launch_bounds__(BLOCK) void k_stage_pack(const float *rho, int ns, int64_t plane_cells, float2 *cbuf) {
+    const int64_t planes = (ns + 1) / 2;
+    for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < planes * plane_cells;
+         t += (int64_t)gridDim.x * BLOCK) {
+        const int64_t p = t / plane_cells, c = t - p * plane_cells;
+        const float re = rho[(2 * p) * plane_cells + c];
+        const float im = 2 * p + 1 < ns ? rho[(2 * p + 1) * plane_cells + c] : 0.0f;
+        cbuf[t] = make_float2(re, im);
+    }
+}
+
+// total_field_1d_numba (src/FFAG_SC_new.py:763-782)
+__global__ __launch_bounds__(BLOCK) void k_stage_total_field_1d(const double *s, const double *q, int n, double *E) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double K = 1.0 / (4.0 * FFA_PI * EPS0);
+    if (n < 2 || s[1] - s[0] <= 0.0) { E[i] = 0.0; return; }
+    const double si = s[i];
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double r = si - s[j];
+        const double sg = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : 0.0);
+        acc += q[j] * sg / (r * r);
+    }
+    E[i] = K * acc;
+}
+
+// interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
+__global__ __launch_bounds__(BLOCK) void k_stage_interp(const float *ex, const float *ez, int ns, int nz, int nx,
+                                                       const double *x, const double *z, const double *pr0,
+                                                       const int32_t *slice, const int64_t *rows, int64_t n, double x0,
+                                                       double z0, double dx, double dz, double bl, double *ex_p,
+                                                       double *ez_p, double *ef_p, double *bunch, int64_t n_rows) {
+    const int64_t k = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int64_t row = rows ? rows[k] : -1;
+    const bool has_row = bunch && row >= 0 && row < n_rows;
+    double vr = 0.0, vz = 0.0, vf = 0.0;
+    if (!((dx <= 0.0) || (dz <= 0.0) || (bl <= 0.0))) {
+        int s = slice[k];
+        s = s < 0 ? 0 : (s >= ns ? ns - 1 : s);
+        const float *EX = ex + (size_t)s * nz * nx, *EZ = ez + (size_t)s * nz * nx;
+        const double t = (x[k] - x0) / dx, u = (z[k] - z0) / dz;
+        const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
+        int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cu);
+        if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
+        if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
+        const double al = t - (double)i0, be = u - (double)j0;
+        const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
+        const int64_t b = j0 * nx + i0;
+        const double er = w00 * EX[b] + w10 * EX[b + 1] + w01 * EX[b + nx] + w11 * EX[b + nx + 1];
+        const double ezv = w00 * EZ[b] + w10 * EZ[b + 1] + w01 * EZ[b + nx] + w11 * EZ[b + nx + 1];
+        const double p = pr0[k];
+        vr = er * (1 / sqrt(1 + p * p));
+        vz = ezv;
+        vf = er * (-p / sqrt(1 + p * p));
+    }
+    ex_p[k] = vr;
+    ez_p[k] = vz;
+    ef_p[k] = vf;
+    if (has_row) {
+        bunch[row * 16 + 10] = vr;
+        bunch[row * 16 + 11] = vz;
+        bunch[row * 16 + 12] = vf;
+    }
+}
+
+inline unsigned grid_cap(int64_t n) {
+    const int64_t g = (n + BLOCK - 1) / BLOCK;
+    return (unsigned)(g < 65536 ? (g > 0 ? g : 1) : 65536);
+}
+
+constexpr size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// Workspace of the stage entry points (regions used by different stages may
+// overlap in time only within one call): sort buffers for n keys, the int64
+// grid, the packed FFT buffer, the FFT tables, partial sums and a result.
+size_t sc_stage_workspace_bytes(int64_t n, int32_t ns, int32_t nz, int32_t nx) {
+    const size_t C = (size_t)ns * nz * nx, P = (size_t)((ns + 1) / 2) * nz * nx;
+    return al256(sort_workspace_bytes(n)) + al256(8 * C) + al256(8 * P) +
+           al256(8 * 4 * (size_t)TW_STRIDE + 4 * (size_t)nz * nx) + al256(sizeof(double) * 16 * NB) + 2 * 256 +
+           al256(4 * (size_t)ns) * 2;
+}
+
+namespace {
+struct stage_ws {
+    char *sort, *grid, *cbuf, *tables, *part, *res, *misc, *ints;
+};
+stage_ws stage_layout(void *ws, int64_t n, int32_t ns, int32_t nz, int32_t nx) {
+    const size_t C = (size_t)ns * nz * nx, P = (size_t)((ns + 1) / 2) * nz * nx;
+    char *w = static_cast<char *>(ws);
+    stage_ws L;
+    L.sort = w; w += al256(sort_workspace_bytes(n));
+    L.grid = w; w += al256(8 * C);
+    L.cbuf = w; w += al256(8 * P);
+    L.tables = w; w += al256(8 * 4 * (size_t)TW_STRIDE + 4 * (size_t)nz * nx);
+    L.part = w; w += al256(sizeof(double) * 16 * NB);
+    L.res = w; w += 256;
+    L.misc = w; w += 256;
+    L.ints = w;
+    return L;
+}
+}  // namespace
+
+hipError_t launch_stage_local_coords(ffa_ctx_s *c, const double *bunch, const uint8_t *survive, int64_t n,
+                                     double speed_c, int rk, double *out, double *count_sum, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, n, 1, 1, 1);   // as ffa_sc_local_coords checks it
+    const double E0J = 938.2723e6 * 1.60217662e-19;   // src/FFAG_SC_new.py:115
+    double *part = reinterpret_cast<double *>(L.part);
+    hipLaunchKernelGGL(k_stage_gamma_partials, dim3(NB), dim3(BLOCK), 0, st, bunch, survive, n, speed_c, rk, E0J, part);
+    hipLaunchKernelGGL(k_stage_reduce_final<2>, dim3(1), dim3(BLOCK), 0, st, part, reduce_ops{{0, 0}}, count_sum);
+    hipError_t e;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_stage_survive_keys, dim3(gridn(n)), dim3(BLOCK), 0, st, survive, n, sort_key_buffer(L.sort));
+        const uint32_t *order;
+        const uint16_t *keys;
+        e = launch_sort_keys16(c, n, 1, L.sort, &order, &keys);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_stage_local_coords, dim3(gridn(n)), dim3(BLOCK), 0, st, bunch, survive, order, n, speed_c,
+                           rk, E0J, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_divide_arc(ffa_ctx_s *c, const double *fi, int64_t n, double *out, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, 0, 1, 1, 1);   // as ffa_sc_divide_arc checks it
+    double *part = reinterpret_cast<double *>(L.part);
+    double *mm = reinterpret_cast<double *>(L.res);
+    hipLaunchKernelGGL(k_stage_arc_partials, dim3(NB), dim3(BLOCK), 0, st, fi, n, part);
+    hipLaunchKernelGGL(k_stage_reduce_final<8>, dim3(1), dim3(BLOCK), 0, st, part, reduce_ops{{1, 1, 1, 1, 2, 2, 2, 2}},
+                       mm);
+    hipLaunchKernelGGL(k_stage_arc_final, dim3(1), dim3(1), 0, st, mm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_slices(ffa_ctx_s *c, const double *r, const double *z, const double *fim, const double *r0,
+                               int64_t n, double step, double fs, int ns, double *x, double *r2d, double *z2d,
+                               double *x2d, int64_t stride, int32_t *counts, int32_t *sl, double *sums, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, n, ns, 1, 1);
+    double *part = reinterpret_cast<double *>(L.part);
+    int32_t *start = reinterpret_cast<int32_t *>(L.ints);
+    hipError_t e;
+    hipLaunchKernelGGL(k_stage_zero_i32, dim3(gridn(ns)), dim3(BLOCK), 0, st, counts, (int64_t)ns);
+    hipLaunchKernelGGL(k_stage_slice_keys, dim3(NB), dim3(BLOCK), 0, st, r, z, fim, r0, n, step, fs, ns, x, sl,
+                       sort_key_buffer(L.sort), counts, part);
+    hipLaunchKernelGGL(k_stage_reduce_final<4>, dim3(1), dim3(BLOCK), 0, st, part, reduce_ops{{0, 0, 0, 0}}, sums);
+    hipLaunchKernelGGL(k_stage_scan_counts, dim3(1), dim3(BLOCK), 0, st, counts, ns, start);
+    if (n > 0) {
+        const uint32_t *order;
+        const uint16_t *keys;
+        e = launch_sort_keys16(c, n, ns - 1, L.sort, &order, &keys);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_stage_slice_place, dim3(gridn(n)), dim3(BLOCK), 0, st, order, keys, start, r, z, x, n,
+                           stride, r2d, z2d, x2d);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_deposit(ffa_ctx_s *c, const double *z2d, const double *x2d, int64_t stride,
+                                const int32_t *counts, int ns, int nz, int nx, double q_macro, double bl, double xmin,
+                                double xmax, double zmin, double zmax, int bits, float *rho, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, 0, ns, nz, nx);
+    unsigned long long *grid = reinterpret_cast<unsigned long long *>(L.grid);
+    const int64_t C = (int64_t)ns * nz * nx;
+    const double dx = (xmax - xmin) / nx, dz = (zmax - zmin) / nz;
+    const bool ok = dx > 0.0 && dz > 0.0 && bl > 0.0;
+    hipLaunchKernelGGL(k_stage_zero_u64, dim3(grid_cap(C)), dim3(BLOCK), 0, st, grid, C);
+    if (ok)
+        hipLaunchKernelGGL(k_stage_deposit, dim3((unsigned)ns), dim3(BLOCK), 0, st, z2d, x2d, stride, counts, nz, nx,
+                           xmin, zmin, 1.0 / dx, 1.0 / dz, bits, grid);
+    const double q = ok ? (q_macro / bl * ns) * (1.0 / (dx * dz)) : 0.0;
+    const double us = q / (double)((int64_t)1 << (2 * bits));
+    hipLaunchKernelGGL(k_stage_grid_to_rho, dim3(grid_cap(C)), dim3(BLOCK), 0, st,
+                       reinterpret_cast<const int64_t *>(grid), C, us, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_green(ffa_ctx_s *c, int nx, int nz, double dx, double dz, float *out, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, 0, 2, nz, nx);
+    ffa_sc_params sp{};
+    sp.ns = 2; sp.nx = nx; sp.nz = nz;
+    ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(L.res);
+    float2 *tw = reinterpret_cast<float2 *>(L.tables);
+    float *green = reinterpret_cast<float *>(L.tables + 8 * 4 * (size_t)TW_STRIDE);
+    hipLaunchKernelGGL(k_stage_set_result, dim3(1), dim3(1), 0, st, res, dx, dz, 1.0, 0.0, 0.0);
+    hipLaunchKernelGGL(k_stage_tables, dim3(grid_cap(4 * TW_STRIDE + (int64_t)nx * nz)), dim3(BLOCK), 0, st, sp, res,
+                       tw, green);
+    hipLaunchKernelGGL(k_stage_green_out, dim3(gridn((int64_t)nx * nz)), dim3(BLOCK), 0, st, green, nz, nx, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_poisson(ffa_ctx_s *c, const float *rho, int ns, int nz, int nx, double dx, double dz, float *phi,
+                                void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, 0, ns, nz, nx);
+    ffa_sc_params sp{};
+    sp.ns = ns; sp.nx = nx; sp.nz = nz;
+    ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(L.res);
+    float2 *tw = reinterpret_cast<float2 *>(L.tables);
+    float *green = reinterpret_cast<float *>(L.tables + 8 * 4 * (size_t)TW_STRIDE);
+    float2 *cbuf = reinterpret_cast<float2 *>(L.cbuf);
+    hipLaunchKernelGGL(k_stage_set_result, dim3(1), dim3(1), 0, st, res, dx, dz, 1.0, 0.0, 0.0);
+    hipLaunchKernelGGL(k_stage_tables, dim3(grid_cap(4 * TW_STRIDE + (int64_t)nx * nz)), dim3(BLOCK), 0, st, sp, res,
+                       tw, green);
+    const int64_t planes = (ns + 1) / 2, pc = (int64_t)nz * nx;
+    hipLaunchKernelGGL(k_stage_pack, dim3(grid_cap(planes * pc)), dim3(BLOCK), 0, st, rho, ns, pc, cbuf);
+    launch_fft_poisson(st, sp, res, nullptr, cbuf, phi, tw, green);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_field(ffa_ctx_s *c, const float *phi, int ns, int nz, int nx, double dr, double dz, double bl,
+                              double mgm1, float *ex, float *ez, void *ws) {
+    hipStream_t st = c->stream;
+    const stage_ws L = stage_layout(ws, 0, 1, 1, 1);   // as ffa_sc_field checks it: only the result block
+    ffa_sc_params sp{};
+    sp.ns = ns; sp.nx = nx; sp.nz = nz;
+    ffa_sc_result *res = reinterpret_cast<ffa_sc_result *>(L.res);
+    hipLaunchKernelGGL(k_stage_set_result, dim3(1), dim3(1), 0, st, res, dr, dz, bl, mgm1, 0.0);
+    launch_field(st, sp, res, phi, nullptr, ex, ez);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_total_field_1d(ffa_ctx_s *c, const double *s, const double *q, int n, double *E) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_total_field_1d, dim3(gridn(n)), dim3(BLOCK), 0, c->stream, s, q, n, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_interp(ffa_ctx_s *c, const float *ex, const float *ez, int ns, int nz, int nx, const double *x,
+                               const double *z, const double *pr0, const int32_t *slice, const int64_t *rows, int64_t n,
+                               double x0, double z0, double dx, double dz, double bl, double *ex_p, double *ez_p,
+                               double *ef_p, double *bunch, int64_t n_rows) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_interp, dim3(gridn(n)), dim3(BLOCK), 0, c->stream, ex, ez, ns, nz, nx, x, z, pr0, slice,
+                       rows, n, x0, z0, dx, dz, bl, ex_p, ez_p, ef_p, bunch, n_rows);
     return hipGetLastError();
 }

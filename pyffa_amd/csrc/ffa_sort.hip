@@ -243,6 +243,52 @@ hipError_t launch_sort_order(ffa_ctx_s *c, const ffa_particles *p, int64_t count
     return hipGetLastError();
 }
 
+uint16_t *sort_key_buffer(void *ws) { return static_cast<uint16_t *>(ws); }
+
+namespace {
+__global__ __launch_bounds__(SBLOCK) void k_sort_iota(int64_t count, uint32_t *__restrict__ idx) {
+    const int64_t i = blockIdx.x * (int64_t)SBLOCK + threadIdx.x;
+    if (i < count) idx[i] = (uint32_t)i;
+}
+}  // namespace
+
+// Stable sort of caller-written 16-bit keys (in sort_key_buffer(ws), count of
+// them): one 8-bit pass when every key is below 256 (max_key), else two.
+// *order_out: the input index of each sorted position; *keys_out: the
+// sorted keys.
+hipError_t launch_sort_keys16(ffa_ctx_s *c, int64_t count, int max_key, void *ws, const uint32_t **order_out,
+                              const uint16_t **keys_out) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char *w = static_cast<char *>(ws);
+    const int64_t nt = sort_tiles(count);
+    uint16_t *k0 = reinterpret_cast<uint16_t *>(w);
+    uint16_t *k1 = reinterpret_cast<uint16_t *>(w + al((size_t)count * 2));
+    uint32_t *i0 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2));
+    uint32_t *i1 = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + al((size_t)count * 4));
+    uint32_t *hist = reinterpret_cast<uint32_t *>(w + 2 * al((size_t)count * 2) + 2 * al((size_t)count * 4));
+    uint32_t *seg = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + al((size_t)nt * 256 * 4));
+    const int64_t total = nt * 256, nseg = (total + SCAN_SEG - 1) / SCAN_SEG;
+    const int passes = max_key < 256 ? 1 : 2;
+    *order_out = passes == 1 ? i1 : i0;
+    *keys_out = passes == 1 ? k1 : k0;
+    if (count == 0) return hipSuccess;
+    const unsigned g = (unsigned)((count + SBLOCK - 1) / SBLOCK);
+    hipLaunchKernelGGL(k_sort_iota, dim3(g), dim3(SBLOCK), 0, c->stream, count, i0);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        uint16_t *ki = pass == 0 ? k0 : k1, *ko = pass == 0 ? k1 : k0;
+        uint32_t *ii = pass == 0 ? i0 : i1, *io = pass == 0 ? i1 : i0;
+        hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, count, shift, (int)nt,
+                           hist);
+        hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nseg), dim3(SBLOCK), 0, c->stream, hist, total, seg);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SBLOCK), 0, c->stream, seg, (int)nseg);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nseg), dim3(SBLOCK), 0, c->stream, hist, total, seg);
+        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)nt), dim3(SBLOCK), 0, c->stream, ki, ii, count, shift,
+                           (int)nt, hist, ko, io);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_particles(ffa_ctx_s *c, const ffa_particles *src, const ffa_particles *dst,
                                    const uint32_t *order, int64_t count, const int64_t *pid_src, int64_t *pid_dst) {
     if (src->n == 0) return hipSuccess;

@@ -69,9 +69,14 @@ def test_workspace_layout_is_aligned_and_ordered():
 def test_invalid_grid_is_rejected_without_gpu():
     lib = _lib.load()
     L = _lib.FfaScLayout()
-    rc = lib.ffa_sc_workspace_layout(100, 16, 48, 64, ctypes.byref(L))
-    assert rc == -1
-    assert b"powers of two" in lib.ffa_last_error(None)
+    # 2-3-5 smooth sizes in [32, 1024] are accepted (160 = 2^5 5, 180 = 2^2 3^2 5, 48 = 2^4 3)
+    for nz, nx in ((160, 160), (180, 160), (48, 64), (1024, 1024)):
+        assert lib.ffa_sc_workspace_layout(100, 16, nz, nx, ctypes.byref(L)) == 0
+    # a prime factor above 5, or a size outside [32, 1024], is refused
+    for nz, nx in ((112, 64), (64, 77), (16, 64), (64, 2048)):
+        rc = lib.ffa_sc_workspace_layout(100, 16, nz, nx, ctypes.byref(L))
+        assert rc == -1
+        assert b"no prime factor above 5" in lib.ffa_last_error(None)
 
 
 def test_missing_gpu_fails_loudly():

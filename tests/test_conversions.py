@@ -1,21 +1,16 @@
-"""Entry/exit state conversions (src/FFAG_ParasAndConversion.py:202-333)
-and the FP32 tier's atan2 polynomial.
+"""Entry/exit state conversions (src/FFAG_ParasAndConversion.py:202-333).
 
 The conversions build every tracking input, so they are pinned by
 relativity (speed from kinetic energy via p c = sqrt(T^2 + 2 T E0)), by the
 reference's definition of pr, pz as angles (tan pr = vr / v_phi,
 tan pz = vz / v_phi) and by the round trip through the inverse conversions.
 """
-import os
-import re
-
 import numpy as np
 import pytest
 
 from pyffa_amd.FFAG_ParasAndConversion import FFAG_ConversionTools
 
 Q, C, E0_MEV = 1.60217662e-19, 2.99792458e8, 938.2723
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _points(n=400, seed=0):
@@ -91,62 +86,3 @@ def test_unit_helpers():
     np.testing.assert_allclose(ct.Ek2P(ek), pc / C, rtol=1e-13)
     np.testing.assert_allclose(ct.v2Ek(_speed(ek)), ek, rtol=1e-9)
     np.testing.assert_allclose(ct.EtotJ2EkMeV(ct.EkMeV2EtotJ(ek)), ek, rtol=1e-12)
-
-
-# ---------------------------------------------------------------------------
-# atan2_f32 of the FP32 push (pyffa_amd/csrc/ffa_fused.hip)
-# ---------------------------------------------------------------------------
-def _atan2_coeffs():
-    src = open(os.path.join(ROOT, "pyffa_amd", "csrc", "ffa_fused.hip")).read()
-    body = src[src.index("float atan2_f32("):]
-    body = body[: body.index("return copysignf")]
-    # p = c7; p = p * s + c6; ... (Horner, highest power first)
-    first = re.search(r"float p = (-?[0-9.e-]+)f;", body).group(1)
-    rest = re.findall(r"p = p \* s ([+-]) ([0-9.e-]+)f;", body)
-    return [np.float32(first)] + [np.float32(("-" if sg == "-" else "") + v) for sg, v in rest]
-
-
-def _atan2_f32(y, x, coeffs):
-    """The kernel's arithmetic in float32: t = min/max in [0, 1], atan t =
-    t P(t^2), octant and quadrant reflections, sign of y."""
-    f = np.float32
-    y, x = y.astype(np.float32), x.astype(np.float32)
-    ax, ay = np.abs(x), np.abs(y)
-    mx, mn = np.maximum(ax, ay), np.minimum(ax, ay)
-    t = (mn * (f(1) / mx)).astype(np.float32)            # the kernel uses the hardware reciprocal
-    s = (t * t).astype(np.float32)
-    p = np.full_like(t, coeffs[0])
-    for c in coeffs[1:]:
-        p = (p * s + c).astype(np.float32)
-    r = (t * p).astype(np.float32)
-    r = np.where(ay > ax, f(1.5707963267948966) - r, r).astype(np.float32)
-    r = np.where(x < 0, f(3.141592653589793) - r, r).astype(np.float32)
-    return np.copysign(r, y)
-
-
-def test_atan2_f32_error_bound():
-    """The degree-7 polynomial P in t^2 (atan t = t P(t^2) on [0, 1]) was
-    fitted by weighted least squares on Chebyshev nodes of t; its FP64 error
-    is 4e-8 rad.  Evaluated in float32, against numpy's float64 atan2 of the
-    same float32 inputs, the error is below 1.5e-7 rad for |angle| < pi/4,
-    2e-7 for |angle| < pi/2 and 3.4e-7 overall: near +-pi the float32
-    result's own spacing is 2.4e-7 and pi - r rounds once more."""
-    coeffs = _atan2_coeffs()
-    assert len(coeffs) == 8
-    ang = np.concatenate([np.linspace(-np.pi, np.pi, 400001), np.arange(-8, 9) * np.pi / 4])
-    for rad in (8.0, 9.3, 1e-3, 1e3):
-        x = (rad * np.cos(ang)).astype(np.float32)
-        y = (rad * np.sin(ang)).astype(np.float32)
-        ok = (x != 0) | (y != 0)
-        got = _atan2_f32(y[ok], x[ok], coeffs).astype(np.float64)
-        want = np.arctan2(y[ok].astype(np.float64), x[ok].astype(np.float64))
-        err = np.abs(np.angle(np.exp(1j * (got - want))))
-        aw = np.abs(want)
-        assert err[aw < np.pi / 4].max() < 1.5e-7, (rad, err.max())
-        assert err[aw < np.pi / 2].max() < 2e-7, (rad, err.max())
-        assert err.max() < 3.4e-7, (rad, err.max())
-    # the same polynomial with its t^2 term's sign flipped misses by far
-    bad = list(coeffs)
-    bad[-2] = -bad[-2]
-    x = np.float32([1.0]); y = np.float32([0.5])
-    assert abs(_atan2_f32(y, x, bad)[0] - np.arctan2(0.5, 1.0)) > 1e-2

@@ -115,12 +115,15 @@ def test_rk4_f64_parity(torch_cuda, tmp_path):
         assert max(err.values()) < tol, (nsteps, err)
 
 
-@pytest.mark.parametrize("grid", [(32, 64, 64), (33, 32, 128), (129, 256, 256)])
+@pytest.mark.parametrize("grid", [(32, 64, 64), (33, 32, 128), (129, 256, 256), (180, 160, 160), (256, 160, 160),
+                                  (33, 48, 96), (336, 160, 160)])
 def test_sc_solve_parity(torch_cuda, grid):
     """One SC solve: scalars, bit-exact fixed-point grid and per-cell counts,
-    potential/fields and particle kicks.  The last grid is above 2^23 cells:
-    the deposit writes slice pairs into the FFT buffer and the gather reads
-    the field cubes."""
+    potential/fields and particle kicks.  (129, 256, 256) and (336, 160, 160)
+    are above 2^23 cells: the deposit writes slice pairs into the FFT buffer
+    and the gather reads the field cubes.  160 = 2^5 5 and 48, 96 = 2^k 3 take
+    the FFT's radix-5 and radix-3 stages ((180, 160, 160) is the grid of the
+    reference's SC benchmarks, benchmarks/test_sc_260508_gauss_backup.py:45-47)."""
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     ns, nz, nx = grid
     print(f"[case] sc solve grid={grid}", flush=True)
