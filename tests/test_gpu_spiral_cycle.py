@@ -1,0 +1,119 @@
+"""The spiral-sector acceleration cycle: 35 degree spiral, order-6 Taylor
+maps from the GPU generator, closed orbits and Twiss from the GPU finders,
+two 200 kV spiral RF gaps on a 300 -> 600 MeV frequency curve, a painted
+injection along an r-up / pz-down curve, driven through main_track.
+
+* At full size (32M macro-particles, (ns, nz, nx) = (1024, 512, 512), FP32
+  tier), where no oracle run fits, through size-independent properties: the
+  whole painted bunch is injected, the fixed-point deposit holds exactly
+  2^(2 bits) units per survivor, slice and cell counts add up to the
+  survivors, a repeated solve is bit-identical.  The painting is compressed
+  in time (4 sub-bunches of 50 ns, 50 ns apart: the whole bunch is in after
+  200 ns instead of 100 turns).
+* At 2000 macro-particles on the same lattice and files (FP64 path), every
+  20-step macro step through the injection of the painted sub-bunches against
+  the oracle restarted from the GPU state; and the synchronous energy gain of
+  the RF over 40 turns."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import cases
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DT = 0.05e-9
+INJ_STEP = 4000                     # tracking starts at -200 ns; injection from t = 0
+
+
+def _oracle_seo(B):
+    return O.SEO(np.asarray(B.SEO_Ek_axis), np.asarray(B.SEO_fi_axis), np.asarray(B.SEO_r_matrix),
+                 np.asarray(B.SEO_pr_matrix), np.asarray(B.SEO_perimeter))
+
+
+def test_spiral_cycle_full_size_properties(tmp_path):
+    from pyffa_amd import FFAG_Distribution as FD
+    from pyffa_amd import main_track, workloads as W
+    from pyffa_amd.FFAG_Bunch import fixed_point_bits
+    n = 32_000_000
+    print(f"[case] spiral cycle full size n={n} grid (1024, 512, 512)", flush=True)
+    path, cfg = W.spiral_cycle_job(str(tmp_path), n_particles=n, grid=(1024, 512, 512, 4.0),
+                                   max_steps=INJ_STEP + 4100, precision="fp32", paint_bunches=4,
+                                   paint_interval_ns=50.0, bunch_length_ns=50.0)
+    print("[case] job written; tracking", flush=True)
+    FD.set_seed(3)
+    rk, B, gp = main_track.main(path)
+    assert B.TotalParticleNum_Local == n and B.InjectedNum_Local == n
+    assert gp.Bmap.max_order == 6 and gp.Emap.GapType == "spiral"
+    fl = B.flags
+    alive = int(((fl & 2) != 0).sum().item())
+    print(f"[case] injected {B.InjectedNum_Local}, alive {alive}", flush=True)
+    assert 0.99 * n < alive <= n
+    for name in ("r", "vr", "z", "vz", "fi", "fidot"):
+        v = getattr(B, name)[(fl & 2) != 0]
+        assert torch.isfinite(v).all(), name
+    B.sc_cell_counts = True
+    B.sc_charge_grid = True
+    bits = fixed_point_bits(n)
+    outs = []
+    for _ in range(2):
+        B.sc_solve_device(first=False, s_last=0.95)
+        res = B.sc_result()
+        assert res["bad_grid"] == 0.0 and res["count"] == alive
+        assert int(B.sc_view("grid").sum().item()) == alive << (2 * bits)
+        assert int(B.sc_view("slice_count").sum().item()) == alive
+        assert int(B.sc_view("cell_count").sum().item()) == alive
+        outs.append((B.esc_pre.clone(), B.sc_view("phi").clone()))     # the new endpoint field
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    assert torch.isfinite(outs[0][1]).all() and outs[0][1].abs().amax().item() > 0.0
+
+
+def test_spiral_cycle_matches_the_oracle_and_accelerates(tmp_path):
+    from pyffa_amd import FFAG_Distribution as FD
+    from pyffa_amd import main_track, workloads as W
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    n = 2000
+    start, stop = INJ_STEP - 20, INJ_STEP + 200
+    print("[case] spiral cycle 2000 particles vs oracle", flush=True)
+    path, cfg = W.spiral_cycle_job(str(tmp_path), n_particles=n, grid=(32, 32, 32, 4.0), max_steps=start,
+                                   precision="fp64", paint_bunches=4, paint_interval_ns=50.0, bunch_length_ns=50.0)
+    FD.set_seed(5)
+    rk, B, gp = main_track.main(path)
+    t = rk.last_run["t"]
+    mc = cases.oracle_machine(gp)
+    assert mc.gap_model == "spiral" and mc.max_order == 6
+    tr = O.Tracker(mc, B.StaticBunchLocal.copy(), enable_sc=True, sc_step=20, grid=tuple(cfg["SC"]["grid_size_rfz"]),
+                   marcosize=cfg["BunchPara"]["ParticleDensity"], seo=_oracle_seo(B))
+    worst = {}
+    for k in range(start // 20, stop // 20):
+        cases.sync_tracker_from_bunch(tr, B)
+        tr.run(t, DT, (k + 1) * 20, start_step=k * 20)
+        lp = dict(stop_condition=dict(max_stepsN=(k + 1) * 20, max_turn=1e9, max_time=1.0), time_step=DT,
+                  step_dumps=None, enable_SC=True, SC_type=1, SC_step=20, verbose=False, start_step=k * 20)
+        rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
+        t = rk.last_run["t"]
+        for c, v in cases.rel_err(B.LocalBunch, tr.M).items():
+            worst[c] = max(worst.get(c, 0.0), v)
+    print("per macro step worst", worst, flush=True)
+    injected = int((B.LocalBunch[:, 7] != 0).sum())
+    assert 0 < injected < n                     # the first sub-bunch is coming in
+    assert max(worst[0], worst[4], worst[9]) < 1e-12, worst
+    assert worst[5] < 1e-11 and worst[1] < 1e-9 and worst[2] < 1e-8 and worst[3] < 1e-7, worst
+    # synchronous acceleration: 2 gaps x 200 kV x sin(30 deg) = 0.2 MeV per
+    # turn for the particles that stay near the synchronous phase
+    from pyffa_amd.FFAG_ParasAndConversion import FFAG_ConversionTools
+    ct = FFAG_ConversionTools()
+    ek0 = ct.ConvertVrzek2Przek_boris(B.LocalBunch)[:, 5]
+    alive0 = B.LocalBunch[:, 8] != 0
+    t_rev = 1.0 / gp.Emap.Interpolation1D_freq_curve(t)[0]
+    turns = 40
+    steps = int(round(turns * t_rev / DT))
+    lp = dict(stop_condition=dict(max_stepsN=stop + steps, max_turn=1e9, max_time=1.0), time_step=DT,
+              step_dumps=None, enable_SC=False, SC_type=1, SC_step=20, verbose=False, start_step=stop)
+    rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
+    M = B.LocalBunch
+    both = alive0 & (M[:, 8] != 0)
+    gain = np.median(ct.ConvertVrzek2Przek_boris(M)[both, 5] - ek0[both])
+    print(f"median energy gain over {turns} turns: {gain:.4f} MeV ({both.sum()} particles)", flush=True)
+    assert abs(gain - 0.2 * turns) < 0.25 * 0.2 * turns

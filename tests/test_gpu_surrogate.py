@@ -189,3 +189,59 @@ def test_push_with_surrogate_tracks_map_push(torch_cuda, trained, tmp_path):
             assert lost_diff < 0.005
     for c in (0, 2):
         assert dev["net"][c] < dev["order0"][c] / 10.0, (c, dev)
+
+
+def test_surrogate_drives_8m_protons_with_space_charge(torch_cuda, trained, tmp_path):
+    """The surrogate workload at its full size: 8M protons (1e13), SC on the
+    (512, 256, 256) grid every 20 steps, RF and aperture, the FP32 tier with
+    the network in place of the order-4 map gather for |z| <= zt and the maps
+    beyond; 100 steps (5 macro steps).  The run is bit-reproducible, the
+    deposit of the final bunch holds exactly 2^(2 bits) units per survivor
+    and the counts add up, and the bunch moments stay within 1e-3 of the
+    FP32 order-4 map tier's on the same input."""
+    torch = torch_cuda
+    from pyffa_amd import workloads as W
+    from pyffa_amd.FFAG_Bunch import FFAG_Bunch, fixed_point_bits
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    n, grid, steps = 8_000_000, (512, 256, 256, 4.0), 100
+    print(f"[case] surrogate 8M with SC, {steps} steps", flush=True)
+    gp = W.global_parameters("stable", max_order=4, rf=dict(v0=100e3), aperture=W.APERTURE, tmpdir=str(tmp_path))
+    M = W.bunch("stable", n=n, seed=0)
+    runs = {}
+    for name, sur in (("net", trained), ("net2", trained), ("map", None)):
+        B = FFAG_Bunch(M, marcosize=1e13 / n, sc_grid_size=grid, BunchType="Boris", precision="fp32")
+        W.set_bunch_seo(B)
+        lp = dict(stop_condition=dict(max_stepsN=steps, max_turn=1e9, max_time=1.0), time_step=0.05e-9,
+                  step_dumps=None, enable_SC=True, SC_type=1, SC_step=20, precision="fp32", verbose=False)
+        if sur is not None:
+            lp["surrogate"] = sur
+        FFAG_RungeKutta().rk4_solve_dt_bunch3_withSC_boris(None, 0.0, B, gp, lp)
+        runs[name] = {k: B.ordered(k) for k in ("r", "vr", "z", "vz", "fi", "fidot", "flags")}
+        if name == "net":
+            alive = int(((B.flags & 2) != 0).sum().item())
+            B.sc_charge_grid = B.sc_cell_counts = True
+            B.sc_solve_device(first=False, s_last=0.95)
+            assert B.sc_result()["count"] == alive
+            assert int(B.sc_view("grid").sum().item()) == alive << (2 * fixed_point_bits(n))
+            assert int(B.sc_view("slice_count").sum().item()) == alive
+            assert int(B.sc_view("cell_count").sum().item()) == alive
+        del B
+    for k in runs["net"]:
+        assert torch.equal(runs["net"][k], runs["net2"][k]), k
+    a, b = runs["net"], runs["map"]
+    both = ((a["flags"] & 2) != 0) & ((b["flags"] & 2) != 0)
+    assert both.sum().item() > 0.99 * n
+
+    def mom(S):
+        r, z = S["r"][both], S["z"][both]
+        v = torch.sqrt(S["vr"][both] ** 2 + S["vz"][both] ** 2 + (r * S["fidot"][both]) ** 2)
+        xp, zp = S["vr"][both] / v, S["vz"][both] / v
+
+        def emit(u, up):
+            u, up = u - u.mean(), up - up.mean()
+            return float(torch.sqrt((u * u).mean() * (up * up).mean() - (u * up).mean() ** 2))
+        return float(r.mean()), float(z.mean()), emit(r, xp), emit(z, zp)
+    ma, mb = mom(a), mom(b)
+    print("net vs map moments", ma, mb, flush=True)
+    assert abs(ma[0] - mb[0]) < 1e-6 and abs(ma[1] - mb[1]) < 1e-6
+    assert abs(ma[2] / mb[2] - 1) < 1e-3 and abs(ma[3] / mb[3] - 1) < 1e-3
