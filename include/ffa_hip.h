@@ -315,6 +315,19 @@ int ffa_sc_interpolate(void *ctx, const float *ex, const float *ez, int32_t ns, 
                        double x0, double z0, double dx, double dz, double bunch_length, double *ex_p, double *ez_p,
                        double *ef_p, double *bunch, int64_t n_rows);
 
+/* ffa_sc_solve without a caller workspace and without any allocation (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a static device array of
+ * FFA_SC_NOALLOC_BYTES per GPU, held from the library's load to the end of
+ * the process; problems whose ffa_sc_workspace_layout total exceeds it are
+ * rejected (FFA_ERR_INVALID).  No region of the array needs clearing
+ * between calls (every region a solve reads, it first writes), but all calls
+ * on one GPU share the array: calls that overlap in time on one GPU, from
+ * any contexts or streams, race.  Same results as ffa_sc_solve. */
+#define FFA_SC_NOALLOC_BYTES ((size_t)256 << 20)
+int ffa_sc_solve_noalloc(void *ctx, const ffa_particles *p, const ffa_sc_params *sp);
+
 /* Stop-condition reduction of src/FFAG_track.py:559-577: out (device, 2
  * doubles) = (sum of non-NaN phi, count of non-NaN phi) over all particles. */
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev);

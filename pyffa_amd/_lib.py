@@ -17,6 +17,7 @@ FFA_MAX_POLY = 8
 FFA_MAX_HARM = 8
 FFA_MAX_GAP = 8
 FFA_SC_BLOCKS = 1024
+SC_NOALLOC_BYTES = 256 << 20
 
 F_INJ = 1
 F_ALIVE = 2
@@ -147,7 +148,7 @@ EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ff
             "ffa_taylor_maps", "ffa_orbit_track", "ffa_spiral_gap_maps",
             "ffa_sc_stage_workspace_size", "ffa_sc_local_coords", "ffa_sc_divide_arc", "ffa_sc_slices",
             "ffa_sc_deposit", "ffa_sc_green", "ffa_sc_poisson", "ffa_sc_field", "ffa_sc_total_field_1d",
-            "ffa_sc_interpolate")
+            "ffa_sc_interpolate", "ffa_sc_solve_noalloc")
 
 
 def load():
@@ -174,6 +175,7 @@ def load():
                                              ctypes.POINTER(_i64), _i32]
     lib.ffa_sc_workspace_layout.argtypes = [_i64, _i32, _i32, _i32, ctypes.POINTER(FfaScLayout)]
     lib.ffa_sc_solve.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams), _vp]
+    lib.ffa_sc_solve_noalloc.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams)]
     lib.ffa_reduce_phi.argtypes = [_vp, ctypes.POINTER(FfaParticles), _vp]
     lib.ffa_bfield_eval.argtypes = [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _d]
     lib.ffa_mlp_workspace_layout.argtypes = [_i64, ctypes.POINTER(FfaMlpLayout)]

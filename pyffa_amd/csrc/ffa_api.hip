@@ -5,8 +5,15 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include "ffa_internal.h"
+
+// Workspace of ffa_sc_solve_noalloc: one array per GPU, allocated when the
+// code object loads and held until the process ends.  Host code never uses
+// this name as an address: each device's address comes from
+// hipGetSymbolAddress, taken once per device in ffa_create.
+static __device__ __align__(256) char scratch[FFA_SC_NOALLOC_BYTES];
 
 int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *L);
 
@@ -51,6 +58,24 @@ int check_particles(ffa_ctx_s *c, const ffa_particles *p) {
 
 ffa_ctx_s *as_ctx(void *v) { return reinterpret_cast<ffa_ctx_s *>(v); }
 
+// Device address of `scratch` on each GPU (looked up once per device, with
+// that device current).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -66,10 +91,16 @@ int ffa_create(int device, void *stream, void **ctx_out) {
     if (!c) return fail(nullptr, FFA_ERR_HIP, "out of host memory");
     c->device = device;
     c->stream = reinterpret_cast<hipStream_t>(stream);
+    if (device >= MAX_DEVICES) {
+        delete c;
+        return fail(nullptr, FFA_ERR_INVALID, "device %d beyond the %d this library handles", device, MAX_DEVICES);
+    }
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->dev), FFA_CTX_DEV_BYTES);
+    if (e == hipSuccess) e = scratch_address(device, &c->scratch);
     if (e != hipSuccess) {
-        int rc = fail(nullptr, FFA_ERR_HIP, "context allocation: %s", hipGetErrorString(e));
+        int rc = fail(nullptr, FFA_ERR_HIP, "context set-up: %s", hipGetErrorString(e));
+        if (c->dev) (void)hipFree(c->dev);
         delete c;
         return rc;
     }
@@ -394,6 +425,32 @@ int ffa_sc_interpolate(void *ctx, const float *ex, const float *ez, int32_t ns, 
     return hip_check(c, launch_stage_interp(c, ex, ez, ns, nz, nx, x, z, pr0, slice, rows, n, x0, z0, dx, dz,
                                             bunch_length, ex_p, ez_p, ef_p, bunch, n_rows),
                      "ffa_sc_interpolate");
+}
+
+int ffa_sc_solve_noalloc(void *ctx, const ffa_particles *p, const ffa_sc_params *sp) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !sp) return fail(c, FFA_ERR_INVALID, "ctx or params is NULL");
+    if (!c->scratch) return fail(c, FFA_ERR_STATE, "the context has no scratch address");
+    if (!(sp->c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed of light must be positive");
+    if (int rc = check_particles(c, p)) return rc;
+    if (sp->ns < 2 || sp->ns > 4096) return fail(c, FFA_ERR_INVALID, "ns must be in [2, 4096]");
+    if (!fft_size_ok(sp->nx) || !fft_size_ok(sp->nz))
+        return fail(c, FFA_ERR_INVALID, "nx, nz must be in [32, 1024] with no prime factor above 5 (got %d, %d)",
+                    sp->nx, sp->nz);
+    if (sp->bits < 1 || sp->bits > 20) return fail(c, FFA_ERR_INVALID, "bits must be in [1, 20]");
+    if ((double)p->n * (double)(1LL << (2 * sp->bits)) >= 9.2e18)
+        return fail(c, FFA_ERR_INVALID, "fixed-point scale 2^%d too large for %lld particles", 2 * sp->bits,
+                    (long long)p->n);
+    if (!sp->seo_r || !sp->seo_pr || !sp->seo_ek_axis || !sp->seo_perimeter || sp->n_ek < 2 || sp->n_fi < 2)
+        return fail(c, FFA_ERR_INVALID, "SEO tables missing or smaller than 2x2");
+    ffa_sc_layout L;
+    ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
+    if (L.total > FFA_SC_NOALLOC_BYTES)
+        return fail(c, FFA_ERR_INVALID,
+                    "ffa_sc_solve_noalloc: %lld particles on a (%d, %d, %d) grid need %zu bytes of workspace, above "
+                    "the %zu of its static array; use ffa_sc_solve with a caller workspace",
+                    (long long)p->n, sp->ns, sp->nz, sp->nx, L.total, (size_t)FFA_SC_NOALLOC_BYTES);
+    return hip_check(c, launch_sc_solve(c, p, sp, c->scratch, &L), "ffa_sc_solve_noalloc");
 }
 
 int ffa_reduce_phi(void *ctx, const ffa_particles *p, double *out_dev) {

@@ -51,6 +51,7 @@ struct ffa_ctx_s {
     char *dev = nullptr;
     size_t dev_bytes = 0;
     const void *mlp_packed = nullptr;   // surrogate weights used by the FP32 push (null: off)
+    void *scratch = nullptr;            // this device's address of the static SC workspace array
 };
 
 static constexpr size_t FFA_CTX_MACHINE_OFF = 0;
