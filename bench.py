@@ -503,9 +503,12 @@ def field_error_leg(gp, n=1_000_000, seed=11):
     b32 = field_at_points(gp, r, fi, z, precision="fp32")
     rms = float(np.sqrt((b64 ** 2).sum(axis=0).mean()))
     rel = np.sqrt(((b32 - b64) ** 2).sum(axis=0)) / rms
+    w = int(np.argmax(rel))
     return {"points": n, "region": "r 8.0-9.3 m, all azimuths, |z| <= 5 cm", "rms_B": rms,
             "rel_rms": float(np.sqrt(np.mean(rel ** 2))), "rel_p99": float(np.quantile(rel, 0.99)),
             "rel_max": float(rel.max()),
+            "worst_point": {"r": float(r[w]), "fi": float(fi[w]), "z": float(z[w]),
+                            "B_fp64": [float(v) for v in b64[:, w]], "B_fp32": [float(v) for v in b32[:, w]]},
             "measure": "|B_fp32 - B_fp64| / rms |B_fp64| per point (Bz, Bphi, Br), as the surrogate's field error"}
 
 

@@ -11,15 +11,14 @@ Same names, arguments and conventions as src/FFAG_Distribution.py:
   one-particle ``getCoordinates()``, each has ``sample(n)`` that draws n
   particles at once (numpy), which the bunch generator uses;
 * ``LoadSEOParams``, ``LoadSEOBetaFunc``, ``GetInitRAndFiForInjection`` (for
-  an injection azimuth or an explicit {fi, r, pr} point),
-  ``GenerateInitDistributionDtInjNew`` and ``GenerateBunches`` with painting
-  (:1152-1741).
+  an injection azimuth, an injection centre line given as a polyline file,
+  or an explicit {fi, r, pr} point), ``GenerateInitDistributionDtInjNew``
+  with the RF-bucket-matched longitudinal type ``match`` (``FFAG_bucket``)
+  and ``GenerateBunches`` with painting (:1152-1741).
 
 Random numbers come from a numpy Generator (``set_seed`` makes a run
-repeatable; the reference draws from the unseeded ``random`` module).  Not
-supported: an injection centre line given as a file (segment intersection)
-and the RF-bucket-matched longitudinal type ``match`` (FFAG_bucket); both
-raise ``NotImplementedError``.
+repeatable; the reference draws from the unseeded ``random`` module and, for
+``match``, from fresh unseeded numpy generators).
 """
 from __future__ import annotations
 
@@ -30,6 +29,7 @@ import re
 import numpy as np
 
 from .FFAG_Bunch import FFAG_ManageBunchAttribute
+from .FFAG_bucket import FFAG_bucket
 
 _rng = np.random.default_rng()
 
@@ -346,12 +346,68 @@ def LoadSEOBetaFunc(SEOPATHName, Ek_c, fi_c):
     return tuple(out)
 
 
+def _polyline_segments(pts):
+    """(n, 2) points -> (n - 1, 4) segments (x1, y1, x2, y2)
+    (CentralLine2Segments, src/FFAG_Utils.py:278-286)."""
+    pts = np.asarray(pts, dtype=np.float64)
+    return np.column_stack((pts[:-1, 0], pts[:-1, 1], pts[1:, 0], pts[1:, 1]))
+
+
+def _segment_crossings(A, B):
+    """Proper crossings of every segment of A with every segment of B
+    (FindIntersectionVect, src/FFAG_Utils.py:32-152): the bounding boxes
+    overlap strictly and each segment's end points lie strictly on either
+    side of the other's line.  Returns the (len(A), len(B)) crossing flags
+    and the signed distances of A's two end points from B's lines."""
+    a1x, a1y, a2x, a2y = (A[:, k][:, None] for k in range(4))
+    b1x, b1y, b2x, b2y = (B[:, k][None, :] for k in range(4))
+    apart = ((np.minimum(a1x, a2x) >= np.maximum(b1x, b2x)) | (np.maximum(a1x, a2x) <= np.minimum(b1x, b2x)) |
+             (np.minimum(a1y, a2y) >= np.maximum(b1y, b2y)) | (np.maximum(a1y, a2y) <= np.minimum(b1y, b2y)))
+    ca1 = (b2x - b1x) * (a1y - b1y) - (b2y - b1y) * (a1x - b1x)
+    ca2 = (b2x - b1x) * (a2y - b1y) - (b2y - b1y) * (a2x - b1x)
+    cb1 = (a2x - a1x) * (b1y - a1y) - (a2y - a1y) * (b1x - a1x)
+    cb2 = (a2x - a1x) * (b2y - a1y) - (a2y - a1y) * (b2x - a1x)
+    lb = np.hypot(b2x - b1x, b2y - b1y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d1 = np.where(lb != 0, ca1 / lb, np.nan)
+        d2 = np.where(lb != 0, ca2 / lb, np.nan)
+    return ~apart & (ca1 * ca2 < 0) & (cb1 * cb2 < 0), d1, d2
+
+
 def GetInitRAndFiForInjection(SEO_R, SEO_Pr, InjectCL, Ek):
-    """(theta_rad, r, pr) of the closed orbit of energy Ek at the injection
-    azimuth InjectCL (degrees)."""
-    if isinstance(InjectCL, np.ndarray):
-        raise NotImplementedError("an injection centre line given as segments is not supported")
+    """(theta_rad, r, pr) of the injection point on the closed orbit of
+    energy Ek.  InjectCL is either the injection azimuth in degrees
+    (cubic Lagrange lookup of r and pr) or an (n, 2) polyline (x, y) [m] of
+    the injection centre line (src/FFAG_Distribution.py:1347-1414): the orbit
+    at Ek (SEO_R / SEO_Pr rows interpolated linearly in Ek) is drawn as a
+    polyline through its azimuth nodes, closed back to its second node as the
+    reference closes it, and the first orbit segment (then the first line
+    segment) that the line crosses gives the point, placed on the orbit
+    segment in the ratio of its end points' distances from the line; theta
+    is its atan2 azimuth in (-pi, pi], r its radius and pr the orbit's pr at
+    the segment's first node.  (The reference's own code for this case ends
+    in an IndexError when it indexes the scalar azimuth, line 1412; this is
+    the computation that code sets up.)"""
     ek_axis, fi_axis = SEO_R[1:, 0], SEO_R[0, 1:]
+    if isinstance(InjectCL, np.ndarray):
+        i = int(np.clip(np.searchsorted(ek_axis, Ek, side="right") - 1, 0, len(ek_axis) - 2))
+        x0, x1 = ek_axis[i], ek_axis[i + 1]
+        w0, w1 = (Ek - x1) / (x0 - x1), (Ek - x0) / (x1 - x0)
+        r_row = w0 * SEO_R[1 + i, 1:] + w1 * SEO_R[2 + i, 1:]
+        pr_row = w0 * SEO_Pr[1 + i, 1:] + w1 * SEO_Pr[2 + i, 1:]
+        xy = np.column_stack((r_row * np.cos(fi_axis), r_row * np.sin(fi_axis)))
+        xy = np.vstack((xy, xy[1]))
+        seo_seg = _polyline_segments(xy)
+        cl = np.atleast_2d(np.asarray(InjectCL, dtype=np.float64))[:, :2]
+        hit, d1, d2 = _segment_crossings(seo_seg, _polyline_segments(cl))
+        ia, ib = np.nonzero(hit)
+        if ia.size == 0:
+            raise ValueError(f"the injection centre line does not cross the closed orbit of Ek = {Ek} MeV")
+        a, b = ia[0], ib[0]
+        pre, post = abs(d1[a, b]), abs(d2[a, b])
+        w = pre / (pre + post)
+        p = (1.0 - w) * seo_seg[a, 0:2] + w * seo_seg[a, 2:4]
+        return float(np.arctan2(p[1], p[0])), float(np.hypot(p[0], p[1])), float(pr_row[a])
     theta = np.array([np.deg2rad(InjectCL)])
     r = lagrange_2d(ek_axis, fi_axis, SEO_R[1:, 1:], np.array([Ek]), theta)[0]
     pr = lagrange_2d(ek_axis, fi_axis, SEO_Pr[1:, 1:], np.array([Ek]), theta)[0]
@@ -373,8 +429,9 @@ def GenerateInitDistributionDtInjNew(SEOFileName, InjectionPosition, Ek_c, Emit,
     seo_pr = np.loadtxt(os.path.join(SEOFileName, "SEO_pr.txt"), skiprows=1)
     Ek_c = seo_ini[0]
     if isinstance(InjectionPosition, str):
-        raise NotImplementedError("an injection centre line file is not supported; give the azimuth in degrees")
-    if isinstance(InjectionPosition, dict):
+        cl = np.loadtxt(InjectionPosition, skiprows=1, ndmin=2)
+        fi_c, r_c, pr_c = GetInitRAndFiForInjection(seo_r, seo_pr, cl, Ek_c)
+    elif isinstance(InjectionPosition, dict):
         fi_c, r_c, pr_c = InjectionPosition["fi"], InjectionPosition["r"], InjectionPosition["pr"]
     elif isinstance(InjectionPosition, (int, float)):
         fi_c, r_c, pr_c = GetInitRAndFiForInjection(seo_r, seo_pr, float(InjectionPosition), Ek_c)
@@ -387,14 +444,20 @@ def GenerateInitDistributionDtInjNew(SEOFileName, InjectionPosition, Ek_c, Emit,
     t_type, l_type = dist_types
     if l_type == "kv":
         eps_l *= 2
-    if l_type == "match":
-        raise NotImplementedError("the RF-bucket-matched longitudinal distribution is not supported")
     tw_r, tw_z = TwissContainer(ar, br, eps_r), TwissContainer(az, bz, eps_z)
     tw_l = TwissContainer(LongitudeAlpha, LongitudeBeta, eps_l)
     dist = CustomDist3D(create_distribution(t_type, twiss_transverse=(tw_r, tw_z)),
                         create_distribution(l_type, twiss_longitudinal=tw_l))
     r, pr, z, pz, t_inj, ek = (np.asarray(v, dtype=np.float64) for v in dist.sample(num_points))
-    if l_type == "UniformGauss":
+    if l_type == "match":
+        # inside the RF bucket, H above the level whose centre-line
+        # trajectory spans LongitudeT ns (src/FFAG_Distribution.py:1275-1280)
+        if bucket_obj is None:
+            bucket_obj = FFAG_bucket(Bmapfoldname=bmap_path, Ek0=Ek_c, fi0=acc_phi0, U0=acc_U0, step_t=20e-9,
+                                     step_N=18000)
+        t_ns, ek_mev, _ = bucket_obj.scatter_filter_by_span(num_points, LongitudeT, show=False, rng=_rng)
+        t_inj, ek = t_ns, ek_mev - Ek_c
+    elif l_type == "UniformGauss":
         t_inj = _rng.uniform(-0.5 * LongitudeT, 0.5 * LongitudeT, num_points)
         ek = _rng.normal(0.0, math.sqrt(eps_l / LongitudeBeta), num_points)
     centre = (r_c + dx_xp_y_yp[0], pr_c + dx_xp_y_yp[1], dx_xp_y_yp[2], dx_xp_y_yp[3], Ek_c, InjTimeNanoSec)

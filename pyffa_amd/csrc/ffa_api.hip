@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 #include "ffa_internal.h"
 
 // Workspace of ffa_sc_solve_noalloc: one array per GPU, allocated when the
@@ -14,6 +15,8 @@
 // this name as an address: each device's address comes from
 // hipGetSymbolAddress, taken once per device in ffa_create.
 static __device__ __align__(256) char scratch[FFA_SC_NOALLOC_BYTES];
+
+static void flutter_table(const ffa_machine &m, std::vector<float> &t);
 
 int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_layout *L);
 
@@ -175,11 +178,19 @@ int ffa_set_machine(void *ctx, const ffa_machine *m) {
     static_assert(FFA_CTX_FCONST_OFF + sizeof(ffa_fconst) <= FFA_CTX_STEPS_OFF, "fconst block overflows");
     ffa_fconst fc;
     ffa_fconst_from_machine(*m, fc);
+    std::vector<float> flut;
+    flutter_table(*m, flut);
+    fc.flut = reinterpret_cast<const float *>(c->dev + FFA_CTX_FLUT_OFF);
+    fc.flut_scale_d = (double)FFA_FLUT_N * (double)m->nsectors / FFA_TWO_PI;
+    fc.flut_scale = (float)fc.flut_scale_d;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->dev + FFA_CTX_MACHINE_OFF, m, sizeof(ffa_machine), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->dev + FFA_CTX_FCONST_OFF, &fc, sizeof(ffa_fconst), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dev + FFA_CTX_FLUT_OFF, flut.data(), flut.size() * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_check(c, e, "ffa_set_machine");
     c->host_machine = *m;
@@ -657,6 +668,41 @@ int ffa_mlp_set(void *ctx, const ffa_mlp_desc *d, const void *packed) {
 
 }  // extern "C"
 
+// The order-0 flutter of one sector for the FP32 tier's table, in FP64 from
+// the same logistic-edge model as the FP64 path (the convex rows: y = y02
+// |y01| s1 s2, s1 = 1 / (1 + e^(-k (x - x0))), s2 = 1 / (1 + e^(k (x - x1))),
+// k = 16 / W0) and its slope F' = sum y k (s2 - s1) at the nodes x_i = i h,
+// h = P / FFA_FLUT_N.  Interval i stores the cubic Hermite interpolant of
+// (F, F') between x_i and x_i+1 as c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1]:
+// one 16-byte load gives F and, through dF/dt / h, F'.
+static void flutter_table(const ffa_machine &m, std::vector<float> &t) {
+    const double P = FFA_TWO_PI / (double)m.nsectors, h = P / (double)FFA_FLUT_N;
+    std::vector<double> F(FFA_FLUT_N + 1), D(FFA_FLUT_N + 1);
+    for (int i = 0; i <= FFA_FLUT_N; ++i) {
+        const double x = (double)i * h;
+        double f = 0.0, f1 = 0.0;
+        for (int j = 0; j < m.n_seg; ++j) {
+            const double x0 = m.convex[j][0], x1 = m.convex[j][1], ks = 16.0 / m.convex[j][2];
+            const double yc = m.convex[j][4] * std::fabs(m.convex[j][3]);
+            const double s1 = 1.0 / (1.0 + std::exp(-ks * (x - x0)));
+            const double s2 = 1.0 / (1.0 + std::exp(ks * (x - x1)));
+            const double y = yc * s1 * s2;
+            f += y;
+            f1 += y * ks * (s2 - s1);
+        }
+        F[i] = f;
+        D[i] = h * f1;                 // dF/dt
+    }
+    t.assign(4 * (size_t)FFA_FLUT_N, 0.0f);
+    for (int i = 0; i < FFA_FLUT_N; ++i) {
+        const double a = F[i], b = F[i + 1], da = D[i], db = D[i + 1];
+        t[4 * (size_t)i + 0] = (float)a;
+        t[4 * (size_t)i + 1] = (float)da;
+        t[4 * (size_t)i + 2] = (float)(3.0 * (b - a) - 2.0 * da - db);
+        t[4 * (size_t)i + 3] = (float)(2.0 * (a - b) + da + db);
+    }
+}
+
 void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {
     std::memset(&f, 0, sizeof f);
     f.T = m.taylor_f32 ? reinterpret_cast<const float *>(m.taylor) : nullptr;
@@ -688,6 +734,10 @@ void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f) {
     f.inv_rstep = m.r_step > 0.0 ? (float)(1.0 / m.r_step) : 0.0f;
     f.fi_min = (float)m.fi_min;
     f.inv_fistep = m.fi_step > 0.0 ? (float)(1.0 / m.fi_step) : 0.0f;
+    f.NXf = (float)m.r_size;
+    f.NYf = (float)m.fi_size;
+    f.NXm2f = (float)(m.r_size - 2);
+    f.NYm2f = (float)(m.fi_size - 2);
     f.n_harm = m.n_harm;
     for (int k = 0; k < m.n_harm; ++k) {
         f.harm[k][0] = (float)m.harm_m[k];

@@ -25,9 +25,8 @@ namespace {
 constexpr int BLOCK = 256;
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-// hardware square root, base-2 logarithm and exponential (about 1 ulp; the
-// FP32 tier's field error budget is far above that)
-__device__ __forceinline__ float hw_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
+// hardware base-2 logarithm and exponential (about 1 ulp; the FP32 tier's
+// field error budget is far above that)
 __device__ __forceinline__ float hw_log2f(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float hw_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
 
@@ -67,63 +66,76 @@ __device__ __forceinline__ void spiral_gap_f32(const ffa_fconst *__restrict__ k,
     gap = hit;
 }
 
+// Radius and inverse radius of an FP32 position: one reciprocal square root.
+__device__ __forceinline__ void radius_f32(float xf, float yf, float &r2, float &rr, float &ir) {
+    r2 = xf * xf + yf * yf;
+    ir = __builtin_amdgcn_rsqf(r2);
+    rr = r2 * ir;
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) f4v gfloat4;
+
 // Field at (r, th, z): th is the azimuth within a turn of the branch cut
 // (harmonics, gap direction, spiral-gap window), fim the same azimuth reduced
 // to the sector [0, P) in FP64 before rounding (the map index and the
 // order-0 field: fast_mod of the reference, src/FFAG_MathTools.py:1553-1557).
-__device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float th, float fim, float z,
-                                           float xf, float yf, float v0, double rf, float &Bz, float &Br,
-                                           float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
+// locd is fim before rounding.  The order-0 flutter F and its slope F' come
+// from the sector table (per-interval cubic Hermite on exact values and
+// slopes: relative error < 1e-7 of max |F| and < 1e-6 of max |F'| at 8192
+// intervals per sector), indexed in FP64 from locd when there is no spiral,
+// so that the steep edges see the FP64 azimuth rather than its FP32
+// rounding.
+__device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float inv_r, float th, float fim,
+                                           double locd, float z, float xf, float yf, float v0, double rf, float &Bz,
+                                           float &Br, float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
                                            bool taylor = true) {
-    float spiral = 0.0f, spiral_p = 0.0f;
-    for (int a = 0; a < k->n_sp; ++a) spiral = spiral * r + k->sp[a];
-    for (int a = 0; a < k->n_spp; ++a) spiral_p = spiral_p * r + k->spp[a];
-    const float inv_r = rcpf(r);
-    float B0r, B0r_p;
-    if (k->kv == 0.0f) {
-        B0r = k->B0;
-        B0r_p = 0.0f;
+    float spiral_p = 0.0f, t;
+    int it;
+    if (k->n_sp > 0) {
+        float spiral = 0.0f;
+        for (int a = 0; a < k->n_sp; ++a) spiral = spiral * r + k->sp[a];
+        float fs = fim - spiral;
+        fs -= k->P * floorf(fs * k->invP);
+        const float u = fmaxf(fs * k->flut_scale, 0.0f);
+        it = min((int)u, FFA_FLUT_N - 1);
+        t = u - (float)it;
     } else {
-        B0r = k->B0 * hw_exp2f(k->kv * hw_log2f(r));
-        B0r_p = B0r * k->kv * inv_r;
+        const double u = fmax(locd * k->flut_scale_d, 0.0);
+        it = min((int)u, FFA_FLUT_N - 1);
+        t = (float)(u - (double)it);
     }
-    float fs = fim - spiral;
-    fs -= k->P * floorf(fs * k->invP);
-    float F = 0.0f, Fp = 0.0f;
-    for (int j = 0; j < k->n_seg; ++j) {
-        const float x0 = k->seg[j][0], x1 = k->seg[j][1], ks = k->seg[j][2], yc = k->seg[j][3];
-        const float k2 = k->seg[j][4];
-        const float s1 = rcpf(1.0f + hw_exp2f(-k2 * (fs - x0)));
-        const float s2 = rcpf(1.0f + hw_exp2f(k2 * (fs - x1)));
-        const float y = yc * s1 * s2;
-        F += y;
-        Fp += y * ks * (s2 - s1);      // ks s1 s2 ((1-s1) - (1-s2))
-    }
+    const f4v fc = *((gfloat4 *)(k->flut) + it);  // global address space: global_load, not flat
+    for (int a = 0; a < k->n_spp; ++a) spiral_p = spiral_p * r + k->spp[a];
+    // B0 r^k (k = 0: exp2(0) = 1, B0 exactly)
+    const float B0r = k->B0 * hw_exp2f(k->kv * hw_log2f(r));
+    const float B0r_p = B0r * k->kv * inv_r;
+    const float F = fmaf(t, fmaf(t, fmaf(t, fc.w, fc.z), fc.y), fc.x);
+    const float Fp = fmaf(t, fmaf(t, 3.0f * fc.w, 2.0f * fc.z), fc.y) * k->flut_scale;
     Bz = B0r * F;
     Bfi = B0r * Fp * inv_r * z;
     Br = (B0r_p * F - B0r * Fp * spiral_p) * z;
     if (k->no >= 1 && taylor) {
-        float fx = (r - k->r_min) * k->inv_rstep;
-        float fy = (fim - k->fi_min) * k->inv_fistep;
-        fx = fminf(fmaxf(fx, -1.0f), (float)k->NX);
-        fy = fminf(fmaxf(fy, -1.0f), (float)k->NY);
-        int i = (int)fx, j = (int)fy;
-        i = i < 0 ? 0 : (i >= k->NX - 1 ? k->NX - 2 : i);
-        j = j < 0 ? 0 : (j >= k->NY - 1 ? k->NY - 2 : j);
+        // cell (int truncation, clamped to [0, N - 2]) and weights of the
+        // position clamped to one cell beyond the map
+        const float fx = __builtin_amdgcn_fmed3f((r - k->r_min) * k->inv_rstep, -1.0f, k->NXf);
+        const float fy = __builtin_amdgcn_fmed3f((fim - k->fi_min) * k->inv_fistep, -1.0f, k->NYf);
+        const int i = (int)__builtin_amdgcn_fmed3f(fx, 0.0f, k->NXm2f);
+        const int j = (int)__builtin_amdgcn_fmed3f(fy, 0.0f, k->NYm2f);
         const float tx = fx - (float)i, ty = fy - (float)j;
         const float w00 = (1.0f - tx) * (1.0f - ty), w01 = (1.0f - tx) * ty, w10 = tx * (1.0f - ty), w11 = tx * ty;
         const int sj = 3 * k->no;
         const float z2 = z * z;
         // azimuth-major map: (i, j), (i+1, j) and (i, j+1), (i+1, j+1) are two
         // contiguous runs of 2 sj floats; the lanes of a wave (one azimuth
-        // cell after the ordering) read neighbouring runs
-        // 32-bit offsets (the host caps the map at 2^32 elements)
+        // cell after the ordering) read neighbouring runs.  32-bit offsets
+        // (the host caps the map at 2^32 elements).
         gfloat *T = (gfloat *)(k->T);          // global address space: global_load, not flat
-        const unsigned o0 = (unsigned)(j * k->NX + i) * (unsigned)sj, o1 = o0 + (unsigned)(k->NX * sj);
-        gfloat *a0 = T + o0, *a1 = T + o1;
+        const unsigned cell = __umul24((unsigned)j, (unsigned)k->NX) + (unsigned)i;
         float ze = z2, zo = z2 * z;
         if (k->no == 2) {
-            // 12 floats per run: order1 (Bz,Bfi,Br), order2 (...), for i and i+1
+            gfloat *a0 = T + cell * 6u, *a1 = a0 + 6u * (unsigned)k->NX;
+            // 12 floats per run: order 1 (Bz, Bfi, Br), order 2, for i and i + 1
             float c0[12], c1[12];
 #pragma unroll
             for (int q = 0; q < 12; ++q) {
@@ -142,7 +154,7 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
                 zo *= z2;
             }
         } else {
-            gfloat *p0 = a0, *p1 = a1;
+            gfloat *p0 = T + cell * (unsigned)sj, *p1 = p0 + (unsigned)(k->NX * sj);
             for (int n = 0; n < k->no; ++n) {
                 gfloat *p00 = p0 + 3 * n, *p10 = p0 + sj + 3 * n, *p01 = p1 + 3 * n, *p11 = p1 + sj + 3 * n;
                 Bz += (w00 * p00[0] + w01 * p01[0] + w10 * p10[0] + w11 * p11[0]) * ze;
@@ -171,11 +183,11 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
             }
             bool inside = true;
             float prev = 0.0f;
-            for (int j = 0; j < 4; ++j) {
-                const float x0 = k->gap_px[g][j], y0 = k->gap_py[g][j];
-                const float x1 = k->gap_px[g][(j + 1) & 3], y1 = k->gap_py[g][(j + 1) & 3];
+            for (int q = 0; q < 4; ++q) {
+                const float x0 = k->gap_px[g][q], y0 = k->gap_py[g][q];
+                const float x1 = k->gap_px[g][(q + 1) & 3], y1 = k->gap_py[g][(q + 1) & 3];
                 const float cross = (x1 - x0) * (yf - y0) - (y1 - y0) * (xf - x0);
-                if (j == 0) prev = cross;
+                if (q == 0) prev = cross;
                 else if (cross * prev < 0.0f) { inside = false; break; }
             }
             if (inside) {
@@ -232,7 +244,11 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     const ET *ep = reinterpret_cast<const ET *>(p.esc_pre);
     const ET *epp = reinterpret_cast<const ET *>(p.esc_prepre);
     const float e0r = (float)ep[ii], e0z = (float)ep[n + ii], e0f = (float)ep[2 * n + ii];
-    const float d_r = e0r - (float)epp[ii], d_z = e0z - (float)epp[n + ii], d_f = e0f - (float)epp[2 * n + ii];
+    // the endpoint slope, zero unless both endpoints exist (the SC flags do
+    // not change inside a launch): E = E_pre + s (E_pre - E_prepre)
+    const bool sc_both = (fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE);
+    const float d_r = sc_both ? e0r - (float)epp[ii] : 0.0f, d_z = sc_both ? e0z - (float)epp[n + ii] : 0.0f;
+    const float d_f = sc_both ? e0f - (float)epp[2 * n + ii] : 0.0f;
     const double r0 = p.r[ii], fi0 = p.fi[ii];
     double s0, c0;
     sincos(fi0, &s0, &c0);
@@ -254,18 +270,24 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     bool moved = false;
     const double q_over_m = q / mm;
     const bool aperture = m.aperture_enable != 0;
+    // first particle of this wave (wave-uniform): the injection test runs
+    // per lane only in waves that overlap the step's injection range
+    const int64_t wave0 = (int64_t)xcd_block() * (SUR ? SUR_BLOCK : BLOCK) +
+                          __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     const double ap0 = m.aperture[0], ap1 = m.aperture[1], ap2 = m.aperture[2], ap3 = m.aperture[3];
     for (int kstep = 0; kstep < nsteps; ++kstep) {
         const ffa_step &s = steps[kstep];
-        if (i >= inj[2 * kstep] && i < inj[2 * kstep + 1]) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
+        const int64_t ilo = inj[2 * kstep], ihi = inj[2 * kstep + 1];
+        if (ilo < wave0 + 64 && ihi > wave0 && i >= ilo && i < ihi) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
         const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
         bool use_sur = false;
         float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
         const float xf = (float)x, yf = (float)y, zf = (float)z;
-        const float r2f = xf * xf + yf * yf;
-        const float rr = hw_sqrtf(r2f);
+        float r2f, rr, ir;
+        radius_f32(xf, yf, r2f, rr, ir);
         const float th = (float)thd;
-        const float fim = (float)(thd - Pd * floor(thd * invPd));
+        const double locd = thd - Pd * floor(thd * invPd);
+        const float fim = (float)locd;
         if constexpr (SUR) {
             float a, u, c;
             mlp::inputs(md, rr, fim, zf, a, u, c);
@@ -280,20 +302,16 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
         }
         if (!live) { rf += s.dRF; continue; }
         moved = true;
-        float esr = e0r, esz = e0z, esf = e0f;
-        if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
-            const float sc = (float)s.sc_s;
-            esr = e0r + sc * d_r; esz = e0z + sc * d_z; esf = e0f + sc * d_f;
-        }
+        const float sc = (float)s.sc_s;
+        const float esr = fmaf(sc, d_r, e0r), esz = fmaf(sc, d_z, e0z), esf = fmaf(sc, d_f, e0f);
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
-        fields_f32(kc, rr, th, fim, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
+        fields_f32(kc, rr, ir, th, fim, locd, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
                    SUR ? (!use_sur && md->fallback != 0) : true);
         if constexpr (SUR) {
             Bz += sBz;
             Bfi += sBfi;
             Br += sBr;
         }
-        const float ir = rcpf(rr);
         const float cf = xf * ir, sf = yf * ir;
         const float ert = Er * srf + esr, eft = Efi * srf + esf, ezt = Ez * srf + esz;
         const double Ex = (double)(ert * cf - eft * sf), Ey = (double)(ert * sf + eft * cf), Ezc = (double)ezt;
@@ -320,7 +338,12 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
         ux = mx + sc2 * (pyp * tz - pzp * ty) + eps * Ex;
         uy = my + sc2 * (pzp * tx - pxp * tz) + eps * Ey;
         uz = mz + sc2 * (pxp * ty - pyp * tx) + eps * Ezc;
-        const double ign = rsqrt(1.0 + (ux * ux + uy * uy + uz * uz) * inv_c2);
+        // 1 / gamma = (1 + u^2 / c^2)^(-1/2): hardware estimate and one
+        // third-order correction (the argument is >= 1 and finite)
+        const double g2 = 1.0 + (ux * ux + uy * uy + uz * uz) * inv_c2;
+        const double y0 = __builtin_amdgcn_rsq(g2);
+        const double eg = fma(-g2 * y0, y0, 1.0);
+        const double ign = fma(y0 * eg, fma(eg, 0.375, 0.5), y0);
         vx = ux * ign; vy = uy * ign; vz = uz * ign;
         x += vx * dt; y += vy * dt; z += vz * dt;
         // azimuth advance: the chord from r to r + v dt turns by dth with
@@ -367,12 +390,14 @@ __global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__r
     sincos(fi[i], &s, &c);
     const double x = r[i] * c, y = r[i] * s;
     const float xf = (float)x, yf = (float)y, zf = (float)z[i];
-    const float rr = hw_sqrtf(xf * xf + yf * yf);
+    float r2f, rr, ir;
+    radius_f32(xf, yf, r2f, rr, ir);
     const double thd = atan2(y, x);
     const double Pd = FFA_TWO_PI / (double)nsectors;
-    const float th = (float)thd, fim = (float)(thd - Pd * floor(thd * ((double)nsectors / FFA_TWO_PI)));
+    const double locd = thd - Pd * floor(thd * ((double)nsectors / FFA_TWO_PI));
+    const float th = (float)thd, fim = (float)locd;
     float Bz, Br, Bfi, Er, Efi, Ez, srf;
-    fields_f32(kc, rr, th, fim, zf, xf, yf, 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
+    fields_f32(kc, rr, ir, th, fim, locd, zf, xf, yf, 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
     out[i] = (double)Bz;
     out[n + i] = (double)Bfi;
     out[2 * n + i] = (double)Br;

@@ -20,6 +20,7 @@ struct ffa_fconst {
     float sp[FFA_MAX_POLY], spp[FFA_MAX_POLY];
     int32_t NX, NY;
     float r_min, inv_rstep, fi_min, inv_fistep;
+    float NXf, NYf, NXm2f, NYm2f;  // NX, NY, NX - 2, NY - 2 (index clamps)
     int32_t n_harm, rf, n_gap, pad;
     float harm[FFA_MAX_HARM][5];   // m, amp, phase, rmin, rmax
     float acc;                     // 1 / gap_width
@@ -33,6 +34,12 @@ struct ffa_fconst {
     float emap_r0, emap_inv_dr, emap_phi0, emap_inv_dphi, emap_phi_max, emap_center;
     float gap_az[FFA_MAX_GAP];
     const double *emap;
+    // order-0 flutter F(fs) over one sector in FFA_FLUT_N intervals of
+    // width h = P / N: per interval the cubic c0 + c1 t + c2 t^2 + c3 t^3 in
+    // t = fs / h - i (device; written by ffa_set_machine)
+    const float *flut;
+    float flut_scale;              // N / P = 1 / h
+    double flut_scale_d;           // N / P in FP64 (index from the FP64 sector azimuth)
 };
 
 void ffa_fconst_from_machine(const ffa_machine &m, ffa_fconst &f);
@@ -48,6 +55,8 @@ struct ffa_ctx_s {
     //   [8K, 8K+256K)  step table for fused launches
     //   [..., +64K)    injection ranges for fused launches
     //   [..., +64K)    reduction partials / scalars
+    //   [..., +4K)     surrogate descriptor
+    //   [..., +128K)   order-0 flutter table of the FP32 tier
     char *dev = nullptr;
     size_t dev_bytes = 0;
     const void *mlp_packed = nullptr;   // surrogate weights used by the FP32 push (null: off)
@@ -63,7 +72,10 @@ static constexpr size_t FFA_CTX_RED_OFF = FFA_CTX_INJ_OFF + FFA_CTX_INJ_BYTES;
 static constexpr size_t FFA_CTX_RED_BYTES = 64 * 1024;
 static constexpr size_t FFA_CTX_MLP_OFF = FFA_CTX_RED_OFF + FFA_CTX_RED_BYTES;   // surrogate descriptor
 static constexpr size_t FFA_CTX_MLP_BYTES = 4096;
-static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_MLP_OFF + FFA_CTX_MLP_BYTES;
+static constexpr int FFA_FLUT_N = 8192;                // flutter table intervals per sector
+static constexpr size_t FFA_CTX_FLUT_OFF = FFA_CTX_MLP_OFF + FFA_CTX_MLP_BYTES;
+static constexpr size_t FFA_CTX_FLUT_BYTES = (size_t)FFA_FLUT_N * 16;
+static constexpr size_t FFA_CTX_DEV_BYTES = FFA_CTX_FLUT_OFF + FFA_CTX_FLUT_BYTES;
 static constexpr int FFA_MAX_FUSED = (int)(FFA_CTX_STEPS_BYTES / sizeof(ffa_step));
 
 static constexpr size_t FFA_CTX_FCONST_OFF = 6144;   // after the machine (< 6 KB)

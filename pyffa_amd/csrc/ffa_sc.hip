@@ -923,34 +923,40 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, floa
 }
 
 // compute_field_from_potential_numba_SBS (src/FFAG_SC_new.py:672-728):
-// float32 differences, float64 scaling, float32 store.  A thread computes
-// FIELD_ROWS consecutive rows of one column (its loads are independent, so
-// their latencies overlap; a wave covers 64 consecutive columns) and stores
-// (Ex, Ez) interleaved, so the gather reads both with one access per corner.
+// float32 differences scaled by float32 factors (the reference scales in
+// float64 before its float32 store; the two agree to an ulp of the store),
+// float32 store.  A thread computes FIELD_ROWS consecutive rows of one column
+// (its loads are independent, so their latencies overlap; a wave covers 64
+// consecutive columns) and stores (Ex, Ez) interleaved, so the gather reads
+// both with one access per corner.
 constexpr int FIELD_ROWS = 4;
 constexpr size_t FROM_PHI_MAX_CELLS = (size_t)1 << 23;   // 32 MB of float32 potential
+
+// Scale factors of the stencils: central differences 1 / (2 d gamma^2),
+// one-sided ones 1 / (d gamma^2) (the reference's x-edge and z-edge rows).
+struct field_scale {
+    float cx1, cx2, cz1, cz2;
+};
+
+__device__ __forceinline__ field_scale make_field_scale(double dr, double dz, double gamma) {
+    const double ig2 = 1.0 / (gamma * gamma);
+    return {(float)(ig2 / dr), (float)(0.5 / dr * ig2), (float)(ig2 / dz), (float)(0.5 / dz * ig2)};
+}
 
 // (Ex, Ez) of cell (j, i) from phi at the cell (pc), above and below it in z
 // (pu, pd: rows clamped to the grid) and left and right of it in x (lf, rt:
 // columns clamped to the row).  Shared by k_sc_field and the gather that
-// differentiates phi itself, so both give the same float32 values.
+// differentiates phi itself, so both give the same float32 values.  At the
+// x edges the x difference is one-sided and the z difference is the
+// clamped-row one scaled by 1 / (dz gamma^2) (the reference's edge rows);
+// on the z edges (not at an x edge) the z difference is one-sided.
 __device__ __forceinline__ float2 cell_field(float pc, float pu, float pd, float lf, float rt, int i, int j, int nx,
-                                             int nz, double dr, double dz, double i2r, double i2z, double ig2) {
-    double ex, ez;
-    if (i == 0 || i == nx - 1) {              // one-sided in x; z neighbours clamped
-        ex = i == 0 ? -(double)(rt - pc) / dr * ig2 : -(double)(pc - lf) / dr * ig2;
-        ez = -(double)(pu - pd) / dz * ig2;
-    } else if (j == 0) {
-        ex = -(double)(rt - lf) * i2r * ig2;
-        ez = -(double)(pu - pc) / dz * ig2;
-    } else if (j == nz - 1) {
-        ex = -(double)(rt - lf) * i2r * ig2;
-        ez = -(double)(pc - pd) / dz * ig2;
-    } else {
-        ex = -(double)(rt - lf) * i2r * ig2;
-        ez = -(double)(pu - pd) * i2z * ig2;
-    }
-    return make_float2((float)ex, (float)ez);
+                                             int nz, const field_scale &f) {
+    const bool xe = i == 0 || i == nx - 1;
+    const bool z0 = !xe && j == 0, z1 = !xe && j == nz - 1;
+    const float dx = i == 0 ? rt - pc : (i == nx - 1 ? pc - lf : rt - lf);
+    const float dz = z0 ? pu - pc : (z1 ? pc - pd : pu - pd);
+    return make_float2(-dx * (xe ? f.cx1 : f.cx2), -dz * ((xe || z0 || z1) ? f.cz1 : f.cz2));
 }
 
 // PLANAR: (Ex, Ez) stored as two (ns, nz, nx) cubes ex, ez (the reference's
@@ -986,9 +992,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
         for (int c = 0; c < rows; ++c) store(j0 + c, make_float2(0.0f, 0.0f));
         return;
     }
-    const double gamma = res->mean_gamma_m1 + 1.0;
-    const double ig2 = 1.0 / (gamma * gamma);
-    const double i2r = 0.5 / dr, i2z = 0.5 / dz;
+    const field_scale fsc = make_field_scale(dr, dz, res->mean_gamma_m1 + 1.0);
     const float *P = phi + base;
     // col[c] = phi(clamp(j0 - 1 + c), i); lf/rt = phi(j, i -+ 1) clamped to the row
     float col[FIELD_ROWS + 2], lf[FIELD_ROWS], rt[FIELD_ROWS];
@@ -1005,7 +1009,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
     }
     for (int c = 0; c < rows; ++c) {
         const int j = j0 + c;
-        store(j, cell_field(col[c + 1], col[c + 2], col[c], lf[c], rt[c], i, j, nx, nz, dr, dz, i2r, i2z, ig2));
+        store(j, cell_field(col[c + 1], col[c + 2], col[c], lf[c], rt[c], i, j, nx, nz, fsc));
     }
 }
 
@@ -1013,9 +1017,9 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
 // 0 <= i0 <= nx - 2 and 0 <= j0 <= nz - 2) straight from phi: the 4 x 4
 // stencil block minus its corners, rows and columns clamped to the grid as
 // k_sc_field clamps them.
-__device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int j0, int nx, int nz, double dr,
-                                                     double dz, double i2r, double i2z, double ig2, float2 &a00,
-                                                     float2 &a10, float2 &a01, float2 &a11) {
+__device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int j0, int nx, int nz,
+                                                     const field_scale &f, float2 &a00, float2 &a10, float2 &a01,
+                                                     float2 &a11) {
     int64_t row[4];
     int col[4];
     for (int k = 0; k < 4; ++k) {
@@ -1027,10 +1031,10 @@ __device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int
     const float v10 = P[row[1] + col[0]], v11 = P[row[1] + col[1]], v12 = P[row[1] + col[2]], v13 = P[row[1] + col[3]];
     const float v20 = P[row[2] + col[0]], v21 = P[row[2] + col[1]], v22 = P[row[2] + col[2]], v23 = P[row[2] + col[3]];
     const float v31 = P[row[3] + col[1]], v32 = P[row[3] + col[2]];
-    a00 = cell_field(v11, v21, v01, v10, v12, i0, j0, nx, nz, dr, dz, i2r, i2z, ig2);
-    a10 = cell_field(v12, v22, v02, v11, v13, i0 + 1, j0, nx, nz, dr, dz, i2r, i2z, ig2);
-    a01 = cell_field(v21, v31, v11, v20, v22, i0, j0 + 1, nx, nz, dr, dz, i2r, i2z, ig2);
-    a11 = cell_field(v22, v32, v12, v21, v23, i0 + 1, j0 + 1, nx, nz, dr, dz, i2r, i2z, ig2);
+    a00 = cell_field(v11, v21, v01, v10, v12, i0, j0, nx, nz, f);
+    a10 = cell_field(v12, v22, v02, v11, v13, i0 + 1, j0, nx, nz, f);
+    a01 = cell_field(v21, v31, v11, v20, v22, i0, j0 + 1, nx, nz, f);
+    a11 = cell_field(v22, v32, v12, v21, v23, i0 + 1, j0 + 1, nx, nz, f);
 }
 
 // interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
@@ -1066,9 +1070,9 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
             const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
             float2 a00, a10, a01, a11;
             if constexpr (FROM_PHI) {
-                const double dr = res->dx, dz = res->dz, gm = res->mean_gamma_m1 + 1.0;
-                cells_field_from_phi(phi + (size_t)s * nz * nx, (int)i0, (int)j0, nx, nz, dr, dz, 0.5 / dr,
-                                     0.5 / dz, 1.0 / (gm * gm), a00, a10, a01, a11);
+                const field_scale fsc = make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0);
+                cells_field_from_phi(phi + (size_t)s * nz * nx, (int)i0, (int)j0, nx, nz, fsc, a00, a10, a01,
+                                     a11);
             } else {
                 const float2 *E = exz + (size_t)s * nz * nx;
                 const int64_t b = j0 * nx + i0;

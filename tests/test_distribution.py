@@ -134,9 +134,11 @@ def test_generate_bunches_painting(tmp_path):
         e_r, _, _ = _twiss_from_moments(S[:, 0], S[:, 1])
         assert abs(e_r / 2e-6 - 1) < 0.03
         assert abs(S[:, 5].std() - np.sqrt(40.0 / 16 / 40.0)) < 0.01          # sqrt(eps_l / beta_l) MeV
-    bp["LongitudeDistType"] = "match"
-    with pytest.raises(NotImplementedError):
-        FD.GenerateBunches(bp)
-    bp["LongitudeDistType"], bp["InjPosition"] = "gauss", "centre_line.cl"
-    with pytest.raises(NotImplementedError):
-        FD.GenerateBunches(bp)
+    # an injection centre line file: a straight line through the orbit at 16.1 deg
+    cl = tmp_path / "centre_line.cl"
+    a = np.deg2rad(16.1)
+    np.savetxt(cl, np.array([[9.0 * np.cos(a), 9.0 * np.sin(a)], [12.0 * np.cos(a), 12.0 * np.sin(a)]]), header="x y")
+    bp["PaintEnable"], bp["InjPosition"] = False, str(cl)
+    M = FD.GenerateBunches(bp)
+    assert M.shape == (20_000, 16)
+    np.testing.assert_allclose(M[:, 4], a, rtol=1e-12)
