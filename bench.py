@@ -300,7 +300,10 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
     path from r and z moved by one ulp: the problem's own sensitivity).  The
     closed-orbit moments are recorded every ``chunk_turns`` turns; the
     relative emittance differences of fp32 and fp64p against fp64 are
-    reported side by side."""
+    reported side by side.  Every run reorders its particles by azimuth
+    every 1000 steps (for the FP64 path this changes only the partition of
+    the SC moment sums, a rounding-level difference like the one fp64p
+    measures)."""
     import numpy as np
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
@@ -328,7 +331,7 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
             last = total * (c + 1) // n_chunks
             lp = dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0), time_step=DT,
                       step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP, precision=prec, verbose=False,
-                      start_step=done * SC_STEP)
+                      start_step=done * SC_STEP, sort_every=PUSHES)
             rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
             t, done = rk.last_run["t"], last
             S = {k: B.ordered(k) for k in ("r", "vr", "z", "vz", "fi", "fidot", "flags")}
@@ -449,7 +452,7 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
         log(f"tune {run}: tracking {n} particles for {total} steps")
         lp = dict(stop_condition=dict(max_stepsN=total, max_turn=1e9, max_time=1.0), time_step=DT,
                   step_dumps=dumps, enable_SC=run != "bare", SC_type=1, SC_step=SC_STEP, precision=prec,
-                  verbose=progress)
+                  verbose=progress, sort_every=PUSHES)
         rk.rk4_solve_dt_bunch3_withSC_boris(None, 0.0, B, gp, lp)
         pd.save_all_particles()
         ids, qx, qz = D.particle_tunes(D.load_position_dump(folder, 0.0), seo, turns=turns)

@@ -264,9 +264,13 @@ class FFAG_FieldSurrogate:
         err = torch.sqrt(((pred - taylor) ** 2).sum(0)) / den
         err0 = torch.sqrt((taylor * taylor).sum(0)) / den
         q = torch.tensor([0.99], dtype=torch.float64, device=self.device)
+        w = int(torch.argmax(err))
+        worst = dict(r=float(r[w]), fi_in_sector_deg=float(np.rad2deg(float(fi[w]) % self.sector)), z=float(z[w]),
+                     B_total=[float(v) for v in total[:, w]], taylor_part=[float(v) for v in taylor[:, w]],
+                     network=[float(v) for v in pred[:, w]])
         return dict(points=n, z_range=float(z_range or self.zt), rms_B=float(den),
                     rel_rms=float(torch.sqrt((err * err).mean())), rel_p99=float(torch.quantile(err[:1_000_000], q)),
-                    rel_max=float(err.max()),
+                    rel_max=float(err.max()), worst_point=worst,
                     order0_rel_rms=float(torch.sqrt((err0 * err0).mean())),
                     order0_rel_p99=float(torch.quantile(err0[:1_000_000], q)), order0_rel_max=float(err0.max()))
 

@@ -123,11 +123,24 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
 // y_t = sum over NCH K chunks of W^T(ob rows, chunk c) x^T(chunk c, tile t).
 // Layers are evaluated block by block so that only two accumulators are
 // live next to the input and output operands (no spills at 256 registers).
-template <int NCH>
+// row of accumulator register r (0..15) in lane half h
+__device__ __forceinline__ int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// BIAS: the accumulators start from the block's biases b (rows ob * 32 ..),
+// so the layer's pre-activations leave the matrix cores complete.
+template <int NCH, bool BIAS = false>
 __device__ __forceinline__ void dense_block(const __bf16 *Wl, int ld, const bf16x8 (&x)[2][8], int ob, int n, int h,
-                                            f32x16 &y0, f32x16 &y1) {
-    y0 = f32x16{};
-    y1 = f32x16{};
+                                            f32x16 &y0, f32x16 &y1, const float *b = nullptr) {
+    if constexpr (BIAS) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            y0[r] = b[ob * 32 + acc_row(r, h)];
+            y1[r] = y0[r];
+        }
+    } else {
+        y0 = f32x16{};
+        y1 = f32x16{};
+    }
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const bf16x8 A = load8(Wl + (ob * 32 + n) * ld + c * 16 + h * 8);
@@ -136,18 +149,11 @@ __device__ __forceinline__ void dense_block(const __bf16 *Wl, int ld, const bf16
     }
 }
 
-// row of accumulator register r (0..15) in lane half h
-__device__ __forceinline__ int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
-
-// bias + SiLU + BF16 of output block ob -> B operands (K chunks 2 ob, 2 ob + 1)
-// of the next layer; y0, y1 keep the pre-activations.
-__device__ __forceinline__ void activate_block(f32x16 &y0, f32x16 &y1, const float *b, int ob, int h,
-                                               bf16x8 (&x)[2][8]) {
+// SiLU + BF16 of the pre-activations (bias included) of output block ob ->
+// B operands (K chunks 2 ob, 2 ob + 1) of the next layer.
+__device__ __forceinline__ void activate_block(const f32x16 &y0, const f32x16 &y1, int ob, bf16x8 (&x)[2][8]) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const float bb = b[ob * 32 + acc_row(r, h)];
-        y0[r] += bb;
-        y1[r] += bb;
         x[0][2 * ob + (r >> 3)][r & 7] = (__bf16)silu(y0[r]);
         x[1][2 * ob + (r >> 3)][r & 7] = (__bf16)silu(y1[r]);
     }
@@ -159,8 +165,8 @@ __device__ __forceinline__ void dense_silu(const __bf16 *Wl, int ld, const float
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
         f32x16 y0, y1;
-        dense_block<NCH>(Wl, ld, xin, ob, n, h, y0, y1);
-        activate_block(y0, y1, b, ob, h, xout);
+        dense_block<NCH, true>(Wl, ld, xin, ob, n, h, y0, y1, b);
+        activate_block(y0, y1, ob, xout);
     }
 }
 

@@ -163,9 +163,10 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const char *__restrict__ pa
         for (int ob = 0; ob < 4; ++ob) {
             f32x16 y[2];
             asm volatile("" : "+v"(nl));   // one block's weight loads at a time
-            if (l == 0) dense_block<4>(Wf + F0, LD0, xi, ob, nl, h, y[0], y[1]);
-            else dense_block<8>(Wf + (l == 1 ? F1 : (l == 2 ? F2 : F3)), LDH, xi, ob, nl, h, y[0], y[1]);
-            activate_block(y[0], y[1], bias + l * HID, ob, h, xo);
+            if (l == 0) dense_block<4, true>(Wf + F0, LD0, xi, ob, nl, h, y[0], y[1], bias);
+            else dense_block<8, true>(Wf + (l == 1 ? F1 : (l == 2 ? F2 : F3)), LDH, xi, ob, nl, h, y[0], y[1],
+                                      bias + l * HID);
+            activate_block(y[0], y[1], ob, xo);
 #pragma unroll
             for (int t = 0; t < 2; ++t)
 #pragma unroll

@@ -722,11 +722,56 @@ __device__ __forceinline__ int idiv(int a, int d, int ld) { return P2 ? (a >> ld
 //   y[q + s (R p + k)] = w^(k p s) sum_j a_j omega_R^(j k),  w = e^(-+2 pi i / N)
 // Every index stays below N.  Lines are ls >= N elements apart (padding
 // against bank conflicts).  Returns the buffer holding the result.
+// Power-of-two sizes: radix-4 stages and at most one radix-2 stage, every
+// index by shifts and masks (integer multiplies and divides are the
+// expensive VALU instructions here).
+__device__ float2 *lds_fft_p2(float2 *x, float2 *y, const float2 *tw, int N, bool inv, int pts, int ls) {
+    const int logN = __builtin_ctz((unsigned)N);
+    const int logq = logN - 2;                           // log2 of the butterflies per line
+    int logs = 0;
+    for (; logs + 2 <= logN; logs += 2) {
+        const int s = 1 << logs, logm = logN - 2 - logs;
+        const int sm = 1 << (logN - 2);                  // s m
+        for (int t = threadIdx.x; t < (pts >> 2); t += BLOCK) {
+            const int line = t >> logq, b = t & ((1 << logq) - 1);
+            const int pp = b >> logs, q = b & (s - 1);
+            const float2 *xl = x + line * ls + q + (pp << logs);
+            float2 *yl = y + line * ls + q + (pp << (logs + 2));
+            const float2 a0 = xl[0], a1 = xl[sm], a2 = xl[2 * sm], a3 = xl[3 * sm];
+            const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
+            const float2 t3 = rot90(csub(a1, a3), inv);
+            const int e = pp << logs;
+            yl[0] = cadd(t0, t2);
+            yl[s] = cmul(cadd(t1, t3), tw[e]);
+            yl[2 * s] = cmul(csub(t0, t2), tw[2 * e]);
+            yl[3 * s] = cmul(csub(t1, t3), tw[3 * e]);
+        }
+        (void)logm;
+        __syncthreads();
+        float2 *tmp = x; x = y; y = tmp;
+    }
+    if (logs < logN) {                                   // radix 2, s = N / 2
+        const int s = 1 << logs, logh = logN - 1;
+        for (int t = threadIdx.x; t < (pts >> 1); t += BLOCK) {
+            const int line = t >> logh, q = t & (s - 1);  // one butterfly group: pp = 0
+            const float2 *xl = x + line * ls + q;
+            float2 *yl = y + line * ls + q;
+            const float2 a0 = xl[0], a1 = xl[s];
+            yl[0] = cadd(a0, a1);
+            yl[s] = cmul(csub(a0, a1), tw[0]);
+        }
+        __syncthreads();
+        float2 *tmp = x; x = y; y = tmp;
+    }
+    return x;
+}
+
 template <bool P2>
 __device__ float2 *lds_fft(float2 *x, float2 *y, const float2 *tw, const fft_plan &pl, bool inv, int pts = FFT_PTS,
                            int ls = 0) {
     const int N = pl.n;
     if (ls == 0) ls = N;
+    if constexpr (P2) return lds_fft_p2(x, y, tw, N, inv, pts, ls);
     const int lines = pts / N;
     int s = 1, logs = 0;
     for (int st = 0; st < pl.nst; ++st) {
@@ -871,8 +916,9 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
     float2 *base = cbuf + plane * (int64_t)nz * nx;
     load_twiddles(tw, twt + 2 * TW_STRIDE, nz);
     load_twiddles(twi, twt + 3 * TW_STRIDE, nz);
+    const int logW = P2 ? __builtin_ctz((unsigned)W) : 0, lognz = P2 ? __builtin_ctz((unsigned)nz) : 0;
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
-        const int j = e / W, c = e - j * W;
+        const int j = idiv<P2>(e, W, logW), c = e - j * W;
         b0[c * LS + j] = c < Wb ? base[(int64_t)j * nx + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
@@ -882,7 +928,7 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
     // transposed table
     const float *gcol = green + (int64_t)c0 * nz;
     for (int e = threadIdx.x; e < Wb * nz; e += BLOCK) {
-        const int c = e / nz, j = e - c * nz;
+        const int c = idiv<P2>(e, nz, lognz), j = e - c * nz;
         const float g = gcol[e];
         float2 &v = o[c * LS + j];
         v = make_float2(v.x * g, v.y * g);
@@ -890,7 +936,7 @@ __global__ __launch_bounds__(BLOCK) void k_fft_cols_green(float2 *cbuf, ffa_sc_p
     __syncthreads();
     float2 *r = lds_fft<P2>(o, other, twi, pl, true, PTS, LS);
     for (int e = threadIdx.x; e < PTS; e += BLOCK) {
-        const int j = e / W, c = e - j * W;
+        const int j = idiv<P2>(e, W, logW), c = e - j * W;
         if (c < Wb) base[(int64_t)j * nx + c0 + c] = r[c * LS + j];
     }
 }
