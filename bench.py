@@ -116,7 +116,9 @@ def accuracy(Sa, Sb, steps):
 def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_every=None, marcosize=None):
     """Tracks the bunch ``warmup`` untimed then ``steps`` timed bench steps
     (PUSHES push steps each) through the tracking loop; returns the bunch,
-    the tracker, the timed seconds and the final state in input order."""
+    the tracker, the timed seconds, the final state in input order and the
+    average kernel time per push launch and per SC solve inside the timed
+    run (HIP events on the launch stream)."""
     import torch
     from pyffa_amd import workloads as W
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
@@ -126,7 +128,7 @@ def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_ev
     W.set_bunch_seo(B)
     rk = FFAG_RungeKutta()
 
-    def lp(first, last):
+    def lp(first, last, timer=None):
         d = dict(stop_condition=dict(max_stepsN=last * PUSHES, max_turn=1e9, max_time=1.0),
                  time_step=DT, step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP,
                  precision=precision, verbose=False, start_step=first * PUSHES)
@@ -134,58 +136,53 @@ def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_ev
             d["surrogate"] = surrogate
         if sort_every is not None:
             d["sort_every"] = sort_every
+        if timer is not None:
+            d["kernel_timer"] = timer
         return d
     t = 0.0
     if warmup:
         rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(0, warmup))
         t = rk.last_run["t"]
+    timer = LaunchTimer(torch.cuda.current_stream())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(warmup, warmup + steps))
+    rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp(warmup, warmup + steps, timer))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return B, rk, el, snapshot(B)
+    return B, rk, el, snapshot(B), timer.result()
 
 
-def kernel_time_ms(rk, B, gp, precision, reps=10):
-    """Average duration of the dominant kernel launch (fused 20-step push for
-    the FP32 tier, one step for FP64) and of one SC solve, with HIP events
-    recorded on the stream the kernels run on."""
-    import torch
-    from pyffa_amd import _lib
-    import numpy as np
-    ctx = rk.ctx
-    stream = torch.cuda.current_stream()
-    assert ctx is not None
-    steps = (_lib.FfaStep * SC_STEP)()
-    inj = (_lib._i64 * (2 * SC_STEP))()
-    t = rk.last_run["t"]
-    for j in range(SC_STEP):
-        f, v0 = gp.Emap.Interpolation1D_freq_curve(t + j * DT)
-        steps[j].t, steps[j].dt, steps[j].dRF, steps[j].v0 = t + j * DT, DT, f * np.pi * 2 * DT, v0
-        steps[j].sc_s = j / SC_STEP
-    res = {}
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ts = []
-    for _ in range(reps):
-        ev[0].record(stream)
-        if precision == "fp32":
-            ctx.check(ctx.lib.ffa_push_boris_fused_f32(ctx.h, B.particles(), steps, inj, SC_STEP), "fused")
-        else:
-            ctx.check(ctx.lib.ffa_push_boris_f64(ctx.h, B.particles(), steps[0]), "f64")
-        ev[1].record(stream)
-        ev[1].synchronize()
-        ts.append(ev[0].elapsed_time(ev[1]))
-    res["push"] = float(np.mean(ts[1:]))
-    ts = []
-    for _ in range(reps):
-        ev[2].record(stream)
-        B.sc_solve_device(ctx, first=False, s_last=0.95)
-        ev[3].record(stream)
-        ev[3].synchronize()
-        ts.append(ev[2].elapsed_time(ev[3]))
-    res["sc"] = float(np.mean(ts[1:]))
-    return res
+class LaunchTimer:
+    """HIP events on the launch stream around every push call and SC solve
+    of a tracking run (the loop's ``kernel_timer`` hook): average duration
+    per kernel launch over the timed region."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.open = {}
+        self.pairs = {"push": [], "sc": []}
+
+    def begin(self, name):
+        import torch
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        self.open[name] = ev
+
+    def end(self, name, launches):
+        import torch
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        self.pairs[name].append((self.open.pop(name), ev, launches))
+
+    def result(self):
+        """{name: average ms per kernel launch} (after the stream is synchronised)."""
+        out = {}
+        for name, pr in self.pairs.items():
+            if pr:
+                out[name] = sum(a.elapsed_time(b) for a, b, _ in pr) / sum(k for _, _, k in pr)
+        out["push_launches"] = sum(k for _, _, k in self.pairs["push"])
+        out["sc_solves"] = len(self.pairs["sc"])
+        return out
 
 
 def cpu_baseline(M, gp, n_sample, blocks, n_total):
@@ -241,14 +238,15 @@ def surrogate_leg(M, tmp, warmup, steps, train_iters):
            "mfma": {"kernel": "k_mlp_eval (4M points)", "ms": t_eval * 1e3, "achieved_tflops": tflops,
                     "peak_tflops": BF16_PEAK_TFLOPS, "frac": tflops / BF16_PEAK_TFLOPS}}
     ps = M.shape[0] * steps * PUSHES
-    B, rk, el_sur, S_sur = run_tier("fp32", M, gp4, warmup, steps, surrogate=sur)
+    B, rk, el_sur, S_sur, k_sur = run_tier("fp32", M, gp4, warmup, steps, surrogate=sur)
     del B, rk
-    B, rk, el_map32, S_map32 = run_tier("fp32", M, gp4, warmup, steps)
+    B, rk, el_map32, S_map32, _ = run_tier("fp32", M, gp4, warmup, steps)
     del B, rk
-    B, rk, el_map64, S_map64 = run_tier("fp64", M, gp4, warmup, steps, sort_every=PUSHES)
+    B, rk, el_map64, S_map64, _ = run_tier("fp64", M, gp4, warmup, steps, sort_every=PUSHES)
     del B, rk
     after = (warmup + steps) * PUSHES
-    out.update({"value": ps / el_sur, "map_order4_fp32_value": ps / el_map32, "map_order4_fp64_value": ps / el_map64,
+    out.update({"value": ps / el_sur, "kernel_ms": k_sur,
+                "map_order4_fp32_value": ps / el_map32, "map_order4_fp64_value": ps / el_map64,
                 "speedup_vs_fp64_order4": el_map64 / el_sur,
                 "accuracy_vs_fp64_order4": accuracy(S_sur, S_map64, after),
                 "fp32_map_accuracy_vs_fp64_order4": accuracy(S_map32, S_map64, after)})
@@ -522,12 +520,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=N_PART)
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-blocks", type=int, default=3, help="SC blocks of 20 steps in the CPU baseline sample")
+    ap.add_argument("--cpu-blocks", type=int, default=6, help="SC blocks of 20 steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed bench steps of the FP64 legs (default --steps)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=2)
+    ap.add_argument("--only-surrogate", action="store_true", help="run the surrogate leg alone")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--accuracy-turns", type=float, default=0.0,
                     help="also run both tiers this many turns and report the moments' deviation (off by default)")
@@ -575,12 +574,15 @@ def main():
         print(json.dumps({"accuracy_long": acc}), flush=True)
         return
     from pyffa_amd import workloads as W
-    gp = machines(2, tmp)
     M = W.bunch("stable", n=args.particles, seed=0)
+    if args.only_surrogate:
+        print(json.dumps({"surrogate": surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters)}),
+              flush=True)
+        return
+    gp = machines(2, tmp)
     n = M.shape[0]
     log(f"inputs ready: {n} particles")
-    B32, rk32, el32, S32 = run_tier("fp32", M, gp, args.warmup, args.steps)
-    k32 = kernel_time_ms(rk32, B32, gp, "fp32")
+    B32, rk32, el32, S32, k32 = run_tier("fp32", M, gp, args.warmup, args.steps)
     log(f"fp32 tier {el32:.3f} s, kernels {k32}")
     del B32, rk32
     ps = n * args.steps * PUSHES
@@ -599,12 +601,10 @@ def main():
     if not args.no_fp64:
         k = args.fp64_steps or args.steps
         ps64 = n * k * PUSHES
-        B64, rk64, el64, S64 = run_tier("fp64", M, gp, args.warmup, k)
-        k64 = kernel_time_ms(rk64, B64, gp, "fp64")
+        B64, rk64, el64, S64, k64 = run_tier("fp64", M, gp, args.warmup, k)
         del B64, rk64
         log(f"fp64 path (input order) {el64:.3f} s, kernels {k64}")
-        B64s, rk64s, el64s, _ = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES)
-        k64s = kernel_time_ms(rk64s, B64s, gp, "fp64")
+        B64s, rk64s, el64s, _, k64s = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES)
         del B64s, rk64s
         log(f"fp64 path (azimuth-ordered) {el64s:.3f} s, kernels {k64s}")
         out["fp64_value"] = ps64 / el64

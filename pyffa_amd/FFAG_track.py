@@ -16,7 +16,13 @@ torch stream, synchronising only for the progress line and the stop check.
 between two SC solves as one fused launch), ``verbose`` (progress lines),
 ``start_step`` (continue a run: the step index the loop starts from, so the
 SC schedule and endpoint history carry on; ``max_stepsN`` stays the absolute
-end) and ``max_block`` (cap on the steps issued per launch block).
+end), ``max_block`` (cap on the steps issued per launch block) and
+``kernel_timer`` (an object whose ``begin(name)`` / ``end(name, launches)``
+the loop calls around each push call, name ``"push"``, with the number of
+kernel launches it issued (1 for a fused FP32 block, one per step on the FP64
+path), and around each SC solve, ``"sc"``, on the host thread that issues
+them: a caller can record stream events there to time the launches inside a
+run).
 """
 from __future__ import annotations
 
@@ -148,6 +154,7 @@ class FFAG_RungeKutta:
         steps_i = int(LocalParameters.get("start_step", 0))
         max_block = int(LocalParameters.get("max_block", _MAX_BLOCK))
         fused = precision == "fp32" and solver == "boris"
+        timer = LocalParameters.get("kernel_timer")
         n_solves = 0
         while steps_i < max_steps:
             # one block: the steps up to the next SC solve / 1000-step check /
@@ -181,7 +188,11 @@ class FFAG_RungeKutta:
                 BunchObj.sort_by_azimuth(ctx)
             if enable_sc and steps_i % sc_step == 0:
                 s_last = ((steps_i - 1) % sc_step) / sc_step if steps_i - 1 >= sc_step else 0.0
+                if timer is not None:
+                    timer.begin("sc")
                 BunchObj.sc_solve_device(ctx, first=(steps_i == 0), s_last=s_last)
+                if timer is not None:
+                    timer.end("sc", 1)
                 n_solves += 1
             steps = (_lib.FfaStep * nb)()
             inj = (_lib._i64 * (2 * nb))()
@@ -208,12 +219,16 @@ class FFAG_RungeKutta:
                 tt = tt + step_t
             if examine:
                 BunchObj.UpdatePreSteps()
+            if timer is not None:
+                timer.begin("push")
             if fused:
                 ctx.check(lib.ffa_push_boris_fused_f32(ctx.h, BunchObj.particles(), steps, inj, nb),
                           "ffa_push_boris_fused_f32")
             else:
                 ctx.check(lib.ffa_push_f64_multi(ctx.h, BunchObj.particles(), steps, inj, nb,
                                                  1 if solver == "rk4" else 0), "ffa_push_f64_multi")
+            if timer is not None:
+                timer.end("push", 1 if fused else nb)
             BunchObj.last_sc_s = steps[nb - 1].sc_s
             BunchObj.UpdatePostSteps()
             if examine:
