@@ -159,14 +159,19 @@ __device__ __forceinline__ void activate_block(const f32x16 &y0, const f32x16 &y
     }
 }
 
+// Software-pipelined over the output blocks: the matrix-core products of
+// block ob + 1 are issued before the SiLU of block ob, so the VALU work of
+// one block runs while the matrix cores compute the next (the two only share
+// the layer's input operands).
 template <int NCH>
 __device__ __forceinline__ void dense_silu(const __bf16 *Wl, int ld, const float *b, const bf16x8 (&xin)[2][8],
                                            bf16x8 (&xout)[2][8], int n, int h) {
+    f32x16 y[2][2];
+    dense_block<NCH, true>(Wl, ld, xin, 0, n, h, y[0][0], y[0][1], b);
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
-        f32x16 y0, y1;
-        dense_block<NCH, true>(Wl, ld, xin, ob, n, h, y0, y1, b);
-        activate_block(y0, y1, ob, xout);
+        if (ob + 1 < 4) dense_block<NCH, true>(Wl, ld, xin, ob + 1, n, h, y[(ob + 1) & 1][0], y[(ob + 1) & 1][1], b);
+        activate_block(y[ob & 1][0], y[ob & 1][1], ob, xout);
     }
 }
 

@@ -1085,74 +1085,120 @@ __device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int
 
 // interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
 // fused with the endpoint roll of src/FFAG_track.py:356-382.
+//
+// GATHER_U particles per thread (i and i + BLOCK within the block's run of
+// GATHER_U * BLOCK): every particle's stream loads are issued together, then
+// every particle's four field pairs, so two dependent round trips are in
+// flight per particle instead of one particle's chain at a time (the field
+// pairs are scattered reads of a cube that stays in the last-level cache).
+constexpr int GATHER_U = 2;
+
 template <typename ET, bool FROM_PHI>
 __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
                                                          const float2 *exz, const float *phi, const double *px,
                                                          const double *ppr0, const int32_t *pslice) {
-    const int64_t i = xcd_block() * (int64_t)BLOCK + threadIdx.x;
-    if (i >= p.n) return;
+    const int64_t base = xcd_block() * (int64_t)(GATHER_U * BLOCK) + threadIdx.x;
     const int64_t n = p.n;
+    if (base >= n) return;
     ET *ep = reinterpret_cast<ET *>(p.esc_pre);
     ET *epp = reinterpret_cast<ET *>(p.esc_prepre);
-    const uint8_t fl = p.flags[i];
-    const bool alive = fl & FFA_F_ALIVE;
     const bool good = res->bad_grid == 0.0;
-    // the old pre-step field is read only where it survives the solve
-    double o_r = 0.0, o_z = 0.0, o_f = 0.0;
-    if (!(alive && good)) { o_r = (double)ep[i]; o_z = (double)ep[n + i]; o_f = (double)ep[2 * n + i]; }
-    double nr = o_r, nz_ = o_z, nf = o_f;
-    if (alive) {
-        if (good) {
-            const int nx = sp.nx, nz = sp.nz;
-            int s = pslice[i];
-            if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-            const double t = (px[i] - res->xmin) * (1.0 / res->dx), u = (p.z[i] - res->zmin) * (1.0 / res->dz);
-            const double pr = ppr0[i];
-            const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cu = u > -1e9 ? (u < 1e9 ? u : 1e9) : -1e9;
-            int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cu);
-            if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
-            if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
-            const double al = t - (double)i0, be = u - (double)j0;
-            const double w00 = (1.0 - al) * (1.0 - be), w10 = al * (1.0 - be), w01 = (1.0 - al) * be, w11 = al * be;
-            float2 a00, a10, a01, a11;
-            if constexpr (FROM_PHI) {
-                const field_scale fsc = make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0);
-                cells_field_from_phi(phi + (size_t)s * nz * nx, (int)i0, (int)j0, nx, nz, fsc, a00, a10, a01,
-                                     a11);
-            } else {
-                const float2 *E = exz + (size_t)s * nz * nx;
-                const int64_t b = j0 * nx + i0;
-                a00 = E[b]; a10 = E[b + 1]; a01 = E[b + nx]; a11 = E[b + nx + 1];
-            }
-            const double er = w00 * a00.x + w10 * a10.x + w01 * a01.x + w11 * a11.x;
-            const double ez = w00 * a00.y + w10 * a10.y + w01 * a01.y + w11 * a11.y;
-            const double ip = 1.0 / sqrt(1.0 + pr * pr);
-            nr = er * ip;
-            nz_ = ez;
-            nf = er * (-pr * ip);
-        } else if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
-            // degenerate grid: the reference leaves LocalBunch columns 10-12 as
-            // the previous step wrote them (the extrapolated field)
-            nr = o_r + sp.s_last * (o_r - (double)epp[i]);
-            nz_ = o_z + sp.s_last * (o_z - (double)epp[n + i]);
-            nf = o_f + sp.s_last * (o_f - (double)epp[2 * n + i]);
+    const int nx = sp.nx, nz = sp.nz;
+    const double xmin = res->xmin, zmin = res->zmin, idx = 1.0 / res->dx, idz = 1.0 / res->dz;
+    // stream loads of every particle (indices past the end read particle base)
+    uint8_t fl[GATHER_U];
+    int sl[GATHER_U];
+    double xv[GATHER_U], zv[GATHER_U], pr[GATHER_U];
+#pragma unroll
+    for (int u = 0; u < GATHER_U; ++u) {
+        const int64_t i = base + u * BLOCK;
+        const int64_t j = i < n ? i : base;
+        fl[u] = p.flags[j];
+        sl[u] = pslice[j];
+        xv[u] = px[j];
+        zv[u] = p.z[j];
+        pr[u] = ppr0[j];
+    }
+    // cells and weights, then every particle's field pairs
+    int64_t cb[GATHER_U];
+    int ci[GATHER_U], cj[GATHER_U];
+    double w00[GATHER_U], w10[GATHER_U], w01[GATHER_U], w11[GATHER_U];
+#pragma unroll
+    for (int u = 0; u < GATHER_U; ++u) {
+        int s = sl[u];
+        if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
+        const double t = (xv[u] - xmin) * idx, v = (zv[u] - zmin) * idz;
+        const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cv = v > -1e9 ? (v < 1e9 ? v : 1e9) : -1e9;
+        int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cv);
+        if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
+        if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
+        const double al = t - (double)i0, be = v - (double)j0;
+        w00[u] = (1.0 - al) * (1.0 - be);
+        w10[u] = al * (1.0 - be);
+        w01[u] = (1.0 - al) * be;
+        w11[u] = al * be;
+        ci[u] = (int)i0;
+        cj[u] = (int)j0;
+        cb[u] = (int64_t)s * nz * nx + j0 * nx + i0;
+    }
+    float2 a00[GATHER_U], a10[GATHER_U], a01[GATHER_U], a11[GATHER_U];
+    if constexpr (FROM_PHI) {
+        const field_scale fsc = make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0);
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t sb = cb[u] - ((int64_t)cj[u] * nx + ci[u]);
+            cells_field_from_phi(phi + sb, ci[u], cj[u], nx, nz, fsc, a00[u], a10[u], a01[u], a11[u]);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            a00[u] = exz[cb[u]];
+            a10[u] = exz[cb[u] + 1];
+            a01[u] = exz[cb[u] + nx];
+            a11[u] = exz[cb[u] + nx + 1];
         }
     }
-    // The two endpoint buffers trade roles at every solve: the new pre-step
-    // field goes into the old prepre memory (the caller then swaps its
-    // pointers) and the old pre memory, untouched, is the new prepre field.
-    // A live particle writes only its new field; a dead one keeps its old.
-    const ET a = (ET)nr, b = (ET)nz_, c = (ET)nf;
-    epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
-    uint8_t nfl;
-    if (sp.first) {
-        ep[i] = a; ep[n + i] = b; ep[2 * n + i] = c;
-        nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | (alive ? FFA_F_SCPRE : 0));
-    } else {
-        nfl = (uint8_t)((fl & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | ((fl & FFA_F_SCPRE) ? FFA_F_SCPREPRE : 0) |
-                        (alive ? FFA_F_SCPRE : 0));
+#pragma unroll
+    for (int u = 0; u < GATHER_U; ++u) {
+        const int64_t i = base + u * BLOCK;
+        if (i >= n) break;
+        const bool alive = fl[u] & FFA_F_ALIVE;
+        // the old pre-step field is read only where it survives the solve
+        double o_r = 0.0, o_z = 0.0, o_f = 0.0;
+        if (!(alive && good)) { o_r = (double)ep[i]; o_z = (double)ep[n + i]; o_f = (double)ep[2 * n + i]; }
+        double nr = o_r, nz_ = o_z, nf = o_f;
+        if (alive) {
+            if (good) {
+                const double er = w00[u] * a00[u].x + w10[u] * a10[u].x + w01[u] * a01[u].x + w11[u] * a11[u].x;
+                const double ez = w00[u] * a00[u].y + w10[u] * a10[u].y + w01[u] * a01[u].y + w11[u] * a11[u].y;
+                const double ip = 1.0 / sqrt(1.0 + pr[u] * pr[u]);
+                nr = er * ip;
+                nz_ = ez;
+                nf = er * (-pr[u] * ip);
+            } else if ((fl[u] & FFA_F_SCPRE) && (fl[u] & FFA_F_SCPREPRE)) {
+                // degenerate grid: the reference leaves LocalBunch columns 10-12
+                // as the previous step wrote them (the extrapolated field)
+                nr = o_r + sp.s_last * (o_r - (double)epp[i]);
+                nz_ = o_z + sp.s_last * (o_z - (double)epp[n + i]);
+                nf = o_f + sp.s_last * (o_f - (double)epp[2 * n + i]);
+            }
+        }
+        // The two endpoint buffers trade roles at every solve: the new pre-step
+        // field goes into the old prepre memory (the caller then swaps its
+        // pointers) and the old pre memory, untouched, is the new prepre field.
+        // A live particle writes only its new field; a dead one keeps its old.
+        const ET a = (ET)nr, b = (ET)nz_, c = (ET)nf;
+        epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
+        uint8_t nfl;
+        if (sp.first) {
+            ep[i] = a; ep[n + i] = b; ep[2 * n + i] = c;
+            nfl = (uint8_t)((fl[u] & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | (alive ? FFA_F_SCPRE : 0));
+        } else {
+            nfl = (uint8_t)((fl[u] & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | ((fl[u] & FFA_F_SCPRE) ? FFA_F_SCPREPRE : 0) |
+                            (alive ? FFA_F_SCPRE : 0));
+        }
+        if (nfl != fl[u]) p.flags[i] = nfl;
     }
-    if (nfl != fl) p.flags[i] = nfl;
 }
 
 inline unsigned gridn(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -1302,7 +1348,7 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     const bool cubes = sp->field_cubes != 0 || C > FROM_PHI_MAX_CELLS;
     if (cubes) launch_field(st, *sp, res, phi, exz, nullptr, nullptr);
     if (p->n > 0) {
-        const dim3 g(gridn(p->n));
+        const dim3 g((unsigned)((p->n + GATHER_U * BLOCK - 1) / (GATHER_U * BLOCK)));
         if (p->esc_f32) {
             if (cubes)
                 hipLaunchKernelGGL((k_sc_gather_roll<float, false>), g, dim3(BLOCK), 0, st, *p, *sp, res, exz, phi, px,
