@@ -25,9 +25,14 @@ Network (same definition as the HIP kernels):
 ``forward``/``backward`` can round to bfloat16 exactly where the GPU does
 (weights, features, hidden activations and the backward signals that feed the
 matrix cores) so that GPU-vs-oracle differences reduce to FP32 summation
-order.  With ``bf16=False`` everything is float64: that mode is what the
-finite-difference gradient check pins.  ``adam_step`` is Algorithm 1 of
-Kingma & Ba (2015).
+order.  In that mode they also follow the GPU's scaled pre-activations:
+the packed W_0 and hidden biases carry KAPPA = -log2(e) and W_4 carries
+1 / KAPPA, so each hidden layer computes y' = KAPPA y and the activation
+a' = y' / (1 + 2^y') = KAPPA silu(y) (BF16-rounded), and the weight gradients
+of layers 1..4, taken from the scaled activations, are divided by KAPPA.
+With ``bf16=False`` everything is float64 and unscaled: that mode is what
+the finite-difference gradient check pins, and the BF16 mode must agree with
+it to BF16 rounding.  ``adam_step`` is Algorithm 1 of Kingma & Ba (2015).
 """
 from __future__ import annotations
 
@@ -90,35 +95,72 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
+KAPPA = np.float32(-1.4426950408889634)        # -log2(e), as the GPU's FP32 constant
+INV_KAPPA = np.float32(-0.6931471805599453)    # -ln(2)
+
+
+def _scales(li, nl):
+    """(weight factor, bias factor, input scale) of layer li in the BF16 mode."""
+    if li == 0:
+        return KAPPA, KAPPA, 1.0
+    if li == nl - 1:
+        return INV_KAPPA, None, KAPPA
+    return None, KAPPA, KAPPA
+
+
 def forward(p, a, u, c2, m, wa, wc, bf=True):
-    """Returns (o, cache); o is (n, 3)."""
-    r = bf16 if bf else (lambda v: np.asarray(v, np.float64))
+    """Returns (o, cache); o is (n, 3).  In the BF16 mode the cache holds
+    the scaled pre-activations KAPPA y and activations BF16(KAPPA silu)."""
     L = unflatten(np.asarray(p, np.float64))
-    x = r(features(a, u, c2, m, wa, wc))
+    if not bf:
+        x = features(a, u, c2, m, wa, wc)
+        xs, zs = [x], []
+        for li, (W, b) in enumerate(L):
+            z = x @ W + b
+            if li == len(L) - 1:
+                return z, (xs, zs, L)
+            zs.append(z)
+            x = silu(z)
+            xs.append(x)
+    x = bf16(features(a, u, c2, m, wa, wc))
     xs, zs = [x], []
     for li, (W, b) in enumerate(L):
-        z = x @ r(W) + b
+        fw, fb, _ = _scales(li, len(L))
+        W32, b32 = W.astype(np.float32), b.astype(np.float32)
+        Wp = bf16(W32 * fw if fw is not None else W32)
+        bp = (b32 * fb if fb is not None else b32).astype(np.float64)
+        z = x @ Wp + bp
         if li == len(L) - 1:
             return z, (xs, zs, L)
         zs.append(z)
-        x = r(silu(z))
+        x = bf16(z / (1.0 + np.exp2(z)))
         xs.append(x)
 
 
 def backward(cache, dy, bf=True):
     """Gradient of sum(dy * o) with respect to the flat parameters."""
-    r = bf16 if bf else (lambda v: np.asarray(v, np.float64))
     xs, zs, L = cache
     g = np.zeros(NPARAM)
-    d = r(dy)
+    if not bf:
+        d = np.asarray(dy, np.float64)
+        for li in range(len(L) - 1, -1, -1):
+            (w, b), (fi, fo) = OFFSETS[li], LAYERS[li]
+            g[w:w + fi * fo] = (xs[li].T @ d).ravel()
+            g[b:b + fo] = d.sum(0)
+            if li == 0:
+                break
+            d = (d @ L[li][0].T) * dsilu(zs[li - 1])
+        return g
+    d = bf16(dy)
     for li in range(len(L) - 1, -1, -1):
         (w, b), (fi, fo) = OFFSETS[li], LAYERS[li]
-        g[w:w + fi * fo] = (xs[li].T @ d).ravel()
+        _, _, xin = _scales(li, len(L))
+        g[w:w + fi * fo] = (xs[li].T @ d).ravel() / float(xin)
         g[b:b + fo] = d.sum(0)
         if li == 0:
             break
-        dx = d @ r(L[li][0]).T
-        d = r(dx * dsilu(zs[li - 1]))
+        dx = d @ bf16(L[li][0]).T
+        d = bf16(dx * dsilu(zs[li - 1] * float(INV_KAPPA)))
     return g
 
 

@@ -15,6 +15,17 @@
 // 3 of the in-chunk index swapped, perm16), so an A fragment is one
 // contiguous 16-byte load.  The backward pass uses the same mapping with W
 // (in x out) in place of W^T.
+//
+// Scaled pre-activations.  The packed forward weights carry the factor
+// KAPPA = -log2(e): W_0 and the hidden biases are multiplied by it and W_4 by
+// 1 / KAPPA, so the matrix cores return y' = KAPPA y for every hidden unit.
+// Then silu(y) = y / (1 + e^-y) is y' / (1 + 2^y') / KAPPA: the activation
+// a' = y' / (1 + 2^y') = KAPPA silu(y) costs an exp2, an add, a reciprocal
+// and one multiply, and the 1 / KAPPA it owes is already in the next layer
+// (hidden W_l are packed unscaled: KAPPA y_(l+1) = W_(l+1) a' + KAPPA b).
+// The backward weights W (in x out) are packed unscaled; the weight
+// gradients taken from the scaled activations are divided by KAPPA when the
+// chunks are summed.
 #pragma once
 #include "ffa_internal.h"
 
@@ -64,11 +75,19 @@ struct fdesc {
     int32_t fallback, pad[3];
 };
 
-__device__ __forceinline__ float silu(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+constexpr float KAPPA = -1.4426950408889634f;          // -log2(e)
+constexpr float INV_KAPPA = -0.6931471805599453f;      // -ln(2)
 
-__device__ __forceinline__ float dsilu(float x) {
-    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-x));
-    return s * (1.0f + x * (1.0f - s));
+// KAPPA silu(y) of a scaled pre-activation yk = KAPPA y (2^yk = e^-y)
+__device__ __forceinline__ float silu_k(float yk) {
+    return yk * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(yk));
+}
+
+// silu'(y) = sigma (1 + y (1 - sigma)) of a scaled pre-activation yk = KAPPA y
+__device__ __forceinline__ float dsilu_k(float yk) {
+    const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(yk));
+    const float y = yk * INV_KAPPA;
+    return s * (1.0f + y * (1.0f - s));
 }
 
 // Taylor-part field of a point from the network outputs o at c = z / zt
@@ -149,13 +168,13 @@ __device__ __forceinline__ void dense_block(const __bf16 *Wl, int ld, const bf16
     }
 }
 
-// SiLU + BF16 of the pre-activations (bias included) of output block ob ->
-// B operands (K chunks 2 ob, 2 ob + 1) of the next layer.
+// SiLU + BF16 of the scaled pre-activations (bias included) of output block
+// ob -> B operands (K chunks 2 ob, 2 ob + 1) of the next layer.
 __device__ __forceinline__ void activate_block(const f32x16 &y0, const f32x16 &y1, int ob, bf16x8 (&x)[2][8]) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        x[0][2 * ob + (r >> 3)][r & 7] = (__bf16)silu(y0[r]);
-        x[1][2 * ob + (r >> 3)][r & 7] = (__bf16)silu(y1[r]);
+        x[0][2 * ob + (r >> 3)][r & 7] = (__bf16)silu_k(y0[r]);
+        x[1][2 * ob + (r >> 3)][r & 7] = (__bf16)silu_k(y1[r]);
     }
 }
 

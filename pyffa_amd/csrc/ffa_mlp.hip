@@ -36,17 +36,18 @@ __device__ __forceinline__ void pack_param(char *packed, int i, float w) {
     float *bias = reinterpret_cast<float *>(packed + FWD_BF16 * 2);
     __bf16 *Wb = reinterpret_cast<__bf16 *>(packed + FWD_BYTES);
     const __bf16 wb = (__bf16)w;
+    // forward copy: W_0 and the hidden biases x KAPPA, W_4 x 1 / KAPPA
     if (i < PB0) {
         const int in = i / HID, out = i % HID;
-        Wf[F0 + out * LD0 + in] = wb;
+        Wf[F0 + out * LD0 + in] = (__bf16)(w * KAPPA);
     } else if (i < PW1) {
-        bias[i - PB0] = w;
+        bias[i - PB0] = w * KAPPA;
     } else if (i < PB3 + HID && i >= PW1) {
         // layers 1..3: [W_l (HID*HID), b_l (HID)] blocks of HID*HID + HID
         const int l = 1 + (i - PW1) / (HID * HID + HID);
         const int j = (i - PW1) % (HID * HID + HID);
         if (j >= HID * HID) {
-            bias[l * HID + (j - HID * HID)] = w;
+            bias[l * HID + (j - HID * HID)] = w * KAPPA;
         } else {
             const int in = j / HID, out = j % HID;
             const int fo = l == 1 ? F1 : (l == 2 ? F2 : F3);
@@ -56,7 +57,7 @@ __device__ __forceinline__ void pack_param(char *packed, int i, float w) {
         }
     } else if (i < PB4) {
         const int j = i - PW4, in = j / NOUT, out = j % NOUT;
-        Wf[F4 + out * LDH + perm16(in)] = wb;
+        Wf[F4 + out * LDH + perm16(in)] = (__bf16)(w * INV_KAPPA);
         Wb[B4 + in * 16 + out] = wb;
     } else {
         bias[4 * HID + (i - PB4)] = w;
@@ -84,6 +85,10 @@ __global__ __launch_bounds__(256) void k_mlp_update(float *__restrict__ params, 
             g = grad_in[i];
         } else {
             for (int s = 0; s < S; ++s) g += partials[(int64_t)s * NPARAM + i];
+            // layers 1..4 took their inputs as the scaled activations KAPPA silu
+            const bool scaled_in = (i >= PW1 && i < PB1) || (i >= PW2 && i < PB2) || (i >= PW3 && i < PB3) ||
+                                   (i >= PW4 && i < PB4);
+            if (scaled_in) g *= INV_KAPPA;
         }
         if (grad_out != nullptr) grad_out[i] = g;
         if (mode == 1) return;      // gradient only: params, moments and packed are not touched (may be null)
@@ -109,7 +114,7 @@ __device__ __forceinline__ void dz_block(const f32x16 (&dx)[2], const float *zw,
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) {
             const float zv = zw[((((l * 2 + t) * 4 + ib) * 16) + rr) * 64 + lane];
-            const __bf16 v = (__bf16)(dx[t][rr] * dsilu(zv));
+            const __bf16 v = (__bf16)(dx[t][rr] * dsilu_k(zv));
             g[t][2 * ib + (rr >> 3)][rr & 7] = v;
             dzfm[(int64_t)(doff(l) + ib * 32 + acc_row(rr, h)) * Bp + pt[t]] = v;
         }

@@ -123,3 +123,33 @@ def test_bf16_forward_close_to_fp64_forward():
     o2, _ = O.forward(p, a, u, c2, m, wa, wc, bf=False)
     rel = np.sqrt(np.mean((o1 - o2) ** 2) / np.mean(o2 ** 2))
     assert 1e-4 < rel < 3e-2
+
+
+def test_scaled_bf16_mode_agrees_with_the_unscaled_float64_mode(monkeypatch):
+    """The BF16 mode's scaled pre-activations (KAPPA = -log2 e folded into
+    the packed W_0, hidden biases and W_4) must give the float64 network's
+    outputs and gradients to BF16 rounding, non-zero biases included; a
+    gradient that forgets to undo the activations' KAPPA fails."""
+    m, wa, wc = S.default_frequencies()
+    a, u, c2 = small_inputs(512, seed=7)
+    rng = np.random.default_rng(13)
+    p = S.init_params(3).astype(np.float64) + rng.normal(0, 0.05, O.NPARAM)
+    t = rng.normal(0, 1, (a.size, 3))
+
+    def grads(bf):
+        o, cache = O.forward(p, a, u, c2, m, wa, wc, bf=bf)
+        _, dy = O.loss_and_dy(o, t)
+        return o, O.backward(cache, dy, bf=bf)
+
+    o64, g64 = grads(False)
+    o16, g16 = grads(True)
+    assert np.sqrt(np.mean((o16 - o64) ** 2) / np.mean(o64 ** 2)) < 3e-2
+    for (w, b), (fi, fo) in zip(O.OFFSETS, O.LAYERS):
+        for sl in (slice(w, w + fi * fo), slice(b, b + fo)):
+            assert np.linalg.norm(g16[sl] - g64[sl]) / np.linalg.norm(g64[sl]) < 5e-2
+    monkeypatch.setattr(O, "_scales", lambda li, nl: (O.KAPPA if li == 0 else (O.INV_KAPPA if li == nl - 1 else None),
+                                                      None if li == nl - 1 else O.KAPPA, 1.0))
+    _, bad = grads(True)
+    w1, _ = O.OFFSETS[1]
+    sl = slice(w1, w1 + 128 * 128)
+    assert np.linalg.norm(bad[sl] - g64[sl]) / np.linalg.norm(g64[sl]) > 0.3
