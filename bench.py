@@ -206,14 +206,14 @@ def cpu_baseline(M, gp, n_sample, blocks, n_total):
                        f"{el:.1f} s")
 
 
-def surrogate_leg(M, tmp, warmup, steps, train_iters):
+def surrogate_leg(M, tmp, warmup, steps, train_iters, zt=0.01):
     """Train the network on the GPU, then run the FP32 tier with it against
     the FP32 and FP64 order-4 map tiers (both azimuth-ordered) on the
     headline workload."""
     import torch
     from pyffa_amd.FFAG_Surrogate import FFAG_FieldSurrogate
     gp4 = machines(4, tmp)
-    sur = FFAG_FieldSurrogate(gp4, zt=0.01)
+    sur = FFAG_FieldSurrogate(gp4, zt=zt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sur.build_dataset(4_000_000, seed=1)
@@ -410,7 +410,8 @@ def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
     return out
 
 
-def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False, grid=GRID):
+def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False, grid=GRID,
+             save_dir=None, load_dir=None):
     """Incoherent tunes with space charge: the bunch of ``accuracy_leg`` is
     tracked ``turns`` + 1.5 turns in each run ("bare": FP64 without SC;
     "fp64", "fp32": with SC; "fp64p": FP64 with SC from r and z moved by one
@@ -418,7 +419,11 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
     the first ``n_track`` particles every turn and the FFT tune estimator
     gives their (qx, qz) about the closed orbit.  The tune shift of a
     particle is its tune with SC minus its bare tune; the shifts of "fp32"
-    and "fp64p" are compared with those of "fp64" particle by particle."""
+    and "fp64p" are compared with those of "fp64" particle by particle.
+    ``save_dir``: each run's per-particle tunes are written there as
+    ``tune_<run>.json``; ``load_dir``: runs saved there by an earlier call
+    (same workload) join the comparison, so the runs can be split over
+    several calls."""
     import numpy as np
     from pyffa_amd import diagnostics as D
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
@@ -433,6 +438,18 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
     out = {"particles": n, "turns": turns, "tracked": n_track, "sc_grid": list(grid[:3]), "steps": total,
            "estimator": "FFT of x + i x'/scale, 8x zero padding, parabolic peak; x about r0(Ek, phi)"}
     q = {}
+    if load_dir:
+        for run in ("bare", "fp64", "fp32", "fp64p"):
+            path = os.path.join(load_dir, f"tune_{run}.json")
+            if run in runs or not os.path.exists(path):
+                continue
+            with open(path) as f:
+                d = json.load(f)
+            qx, qz = np.asarray(d["qx"]), np.asarray(d["qz"])
+            q[run] = dict(zip(d["ids"], zip(d["qx"], d["qz"])))
+            out[run] = {"seconds": d["seconds"], "particles_analysed": len(d["ids"]), "loaded_from": path,
+                        "qx_mean": float(qx.mean()), "qz_mean": float(qz.mean()),
+                        "qx_rms": float(qx.std()), "qz_rms": float(qz.std())}
     for run in runs:
         prec = "fp32" if run == "fp32" else "fp64"
         M = M0.copy()
@@ -460,10 +477,15 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
                     "qx_rms": float(qx.std()), "qz_rms": float(qz.std())}
         log(f"tune {run}: {ids.size} particles, qx {qx.mean():.5f} qz {qz.mean():.5f}, "
             f"{out[run]['seconds']:.0f} s")
+        if save_dir:
+            os.makedirs(save_dir, exist_ok=True)
+            with open(os.path.join(save_dir, f"tune_{run}.json"), "w") as f:
+                json.dump({"ids": ids.tolist(), "qx": qx.tolist(), "qz": qz.tolist(),
+                           "seconds": out[run]["seconds"], "particles": n, "turns": turns}, f)
         del B, rk
     if "bare" in q:
         shifts = {}
-        for run in runs:
+        for run in q:
             if run == "bare":
                 continue
             common = sorted(set(q[run]) & set(q["bare"]))
@@ -527,6 +549,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=2)
     ap.add_argument("--only-surrogate", action="store_true", help="run the surrogate leg alone")
+    ap.add_argument("--surrogate-zt", type=float, default=0.01, help="|z| range [m] the network is trained on")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--accuracy-turns", type=float, default=0.0,
                     help="also run both tiers this many turns and report the moments' deviation (off by default)")
@@ -543,6 +566,9 @@ def main():
     ap.add_argument("--tune-tracked", type=int, default=2000)
     ap.add_argument("--tune-grid", default=None, help="SC grid ns,nx,nz of the tune leg")
     ap.add_argument("--only-tune", action="store_true", help="run the tune leg alone")
+    ap.add_argument("--tune-runs", default="bare,fp64,fp32,fp64p", help="runs of the tune leg in this call")
+    ap.add_argument("--tune-save", default=None, help="directory for each run's per-particle tunes")
+    ap.add_argument("--tune-load", default=None, help="directory of runs saved by an earlier call")
     ap.add_argument("--tune-progress", action="store_true",
                     help="the tune leg's tracking prints its progress lines (on stdout, before the JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -565,8 +591,9 @@ def main():
         return
     if args.only_tune:
         print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
-                                                 progress=args.tune_progress, grid=_grid_arg(args.tune_grid))}),
-              flush=True)
+                                                 runs=tuple(args.tune_runs.split(",")), progress=args.tune_progress,
+                                                 grid=_grid_arg(args.tune_grid), save_dir=args.tune_save,
+                                                 load_dir=args.tune_load)}), flush=True)
         return
     if args.only_accuracy:
         acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","),
@@ -576,8 +603,8 @@ def main():
     from pyffa_amd import workloads as W
     M = W.bunch("stable", n=args.particles, seed=0)
     if args.only_surrogate:
-        print(json.dumps({"surrogate": surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters)}),
-              flush=True)
+        print(json.dumps({"surrogate": surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters,
+                                                     args.surrogate_zt)}), flush=True)
         return
     gp = machines(2, tmp)
     n = M.shape[0]
@@ -633,7 +660,7 @@ def main():
                        "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"],
                        "valu": measured_valu("k_boris_fused<float, false>")}
     if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters)
+        out["surrogate"] = surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters, args.surrogate_zt)
     if not args.no_mapgen:
         out["map_generator"] = mapgen_leg(cpu=not args.no_cpu)
     if args.tune_turns > 0:

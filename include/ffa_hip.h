@@ -130,7 +130,9 @@ typedef struct {
                               the kicks from them; 0: the library chooses, and on grids
                               of up to 2^23 cells the gather differentiates phi at the
                               particle's four cells itself (same arithmetic, same kicks)
-                              and leaves the exz region unwritten */
+                              and leaves the exz region unwritten, while above 2^23
+                              cells it stores the cubes and leaves the phi region
+                              unwritten (phi never leaves LDS) */
     /* SEO tables, device, row-major (n_ek, n_fi) */
     const double *seo_r, *seo_pr;
     const double *seo_ek_axis, *seo_perimeter;   /* n_ek each */
@@ -158,7 +160,8 @@ typedef struct {
     size_t grid;        /* int64 (ns, nz, nx) fixed-point charge (when charge_grid asks) */
     size_t cell_count;  /* int32 (ns, nz, nx) particles per CIC base cell */
     size_t cbuf;        /* complex64 (ceil(ns/2), nz, nx) packed slice pairs */
-    size_t phi;         /* float (ns, nz, nx) potential */
+    size_t phi;         /* float (ns, nz, nx) potential (not written when the library
+                           chooses the field cubes itself: field_cubes = 0 above 2^23 cells) */
     size_t exz;         /* float2 (ns, nz, nx): (Ex, Ez) of each cell, interleaved */
     size_t px;          /* double n: x = r - r0 */
     size_t ppr0;        /* double n: pr0 */

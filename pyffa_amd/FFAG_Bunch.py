@@ -445,8 +445,10 @@ class FFAG_Bunch:
     def sc_view(self, name):
         """Typed device view of one workspace region (after a solve); "ex"
         and "ez" are the components of the interleaved field region, written
-        only by solves with ``sc_field_cubes`` set; "grid" is written only by
-        solves with ``sc_charge_grid`` set."""
+        by solves with ``sc_field_cubes`` set (and by every solve on grids
+        above 2^23 cells); "phi" is written by every solve except those on
+        grids above 2^23 cells without ``sc_field_cubes``; "grid" is written
+        only by solves with ``sc_charge_grid`` set."""
         t = self.torch
         if name in ("ex", "ez"):
             return self.sc_view("exz")[..., 0 if name == "ex" else 1]

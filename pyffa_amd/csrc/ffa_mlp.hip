@@ -417,8 +417,13 @@ hipError_t launch_mlp_eval(ffa_ctx_s *c, const ffa_mlp_desc *d, const void *pack
                            float *out, int mode) {
     if (n == 0) return hipSuccess;
     const fdesc fd = mlp::fdesc_from(*d);
+    // the weights take 133 KB of LDS, so one workgroup fits a CU: launch one
+    // per CU and let each loop over its batches, loading the weights once
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e != hipSuccess) return e;
     int64_t g = (n + EVAL_BLOCK - 1) / EVAL_BLOCK;
-    if (g > 4096) g = 4096;
+    if (g > ncu) g = ncu > 0 ? ncu : 1;
     hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)g), dim3(EVAL_BLOCK), 0, c->stream,
                        static_cast<const char *>(packed), fd, pts, n, out, mode);
     return hipGetLastError();

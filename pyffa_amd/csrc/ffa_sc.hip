@@ -1059,6 +1059,68 @@ __global__ __launch_bounds__(BLOCK) void k_sc_field(const float *phi, float2 *ex
     }
 }
 
+// Inverse row FFTs fused with compute_field_from_potential_numba_SBS (grids
+// whose field cubes are stored): a workgroup transforms the rows of one
+// slice pair's band [j0, j0 + R) plus the rows j0 - 1 and j0 + R (clamped to
+// the grid, as the stencils clamp them), normalises them to phi in LDS and
+// writes the (Ex, Ez) cube of the band's rows with k_sc_field's stencils
+// (cell_field), so phi need not make a round trip through HBM.  phi itself
+// is stored when `phi` is not null.  Same arithmetic as k_fft_rows_inv
+// followed by k_sc_field: bit-identical cubes.
+__host__ __device__ inline int field_band_lines(int nx) { return 2048 / nx > 4 ? 2048 / nx : 4; }
+
+template <bool P2>
+__global__ __launch_bounds__(BLOCK) void k_fft_rows_inv_field(const float2 *cbuf, float *phi, float2 *exz,
+                                                              ffa_sc_params sp, const ffa_sc_result *res,
+                                                              const float2 *twt, fft_plan pl, int lognx) {
+    extern __shared__ float2 lds2[];           // b0, b1 [lines * nx], tw[nx]
+    const int nx = sp.nx, nz = sp.nz, ns = sp.ns;
+    const int L = field_band_lines(nx), R = L - 2;
+    const int nbands = (nz + R - 1) / R;
+    const int plane = (int)(blockIdx.x / nbands), band = (int)(blockIdx.x - (unsigned)plane * nbands);
+    const int j0 = band * R, rows = min(R, nz - j0);
+    const int pts = (rows + 2) * nx;
+    float2 *b0 = lds2, *b1 = lds2 + L * nx, *tw = lds2 + 2 * L * nx;
+    load_twiddles(tw, twt + TW_STRIDE, nx);
+    const float2 *src = cbuf + (int64_t)plane * nz * nx;
+    for (int e = threadIdx.x; e < pts; e += BLOCK) {
+        const int l = idiv<P2>(e, nx, lognx), i = e - l * nx;
+        int j = j0 - 1 + l;
+        j = j < 0 ? 0 : (j > nz - 1 ? nz - 1 : j);
+        b0[e] = src[(int64_t)j * nx + i];
+    }
+    __syncthreads();
+    float2 *o = lds_fft<P2>(b0, b1, tw, pl, true, pts);
+    const float norm = 1.0f / (float)(nx * nz);
+    for (int e = threadIdx.x; e < pts; e += BLOCK) o[e] = make_float2(o[e].x * norm, o[e].y * norm);
+    __syncthreads();
+    const int64_t sa = 2 * (int64_t)plane, sb = sa + 1;
+    const bool two = sb < ns;
+    if (phi != nullptr)
+        for (int e = threadIdx.x; e < rows * nx; e += BLOCK) {
+            const int l = idiv<P2>(e, nx, lognx), i = e - l * nx;
+            const float2 v = o[(l + 1) * nx + i];
+            phi[(sa * nz + j0 + l) * nx + i] = v.x;
+            if (two) phi[(sb * nz + j0 + l) * nx + i] = v.y;
+        }
+    const double dr = res->dx, dz = res->dz, bl = res->bunch_length;
+    const bool zero = res->bad_grid != 0.0 || dr == 0.0 || dz == 0.0 || bl == 0.0;
+    const field_scale fsc = zero ? field_scale{} : make_field_scale(dr, dz, res->mean_gamma_m1 + 1.0);
+    for (int e = threadIdx.x; e < rows * nx; e += BLOCK) {
+        const int l = idiv<P2>(e, nx, lognx), i = e - l * nx, j = j0 + l;
+        const float2 *c = o + (l + 1) * nx;
+        const float2 pc = c[i], pu = c[nx + i], pd = c[i - nx];
+        const float2 lf = c[i > 0 ? i - 1 : 0], rt = c[i < nx - 1 ? i + 1 : nx - 1];
+        float2 fa = make_float2(0.0f, 0.0f), fb = fa;
+        if (!zero) {
+            fa = cell_field(pc.x, pu.x, pd.x, lf.x, rt.x, i, j, nx, nz, fsc);
+            fb = cell_field(pc.y, pu.y, pd.y, lf.y, rt.y, i, j, nx, nz, fsc);
+        }
+        exz[(sa * nz + j) * nx + i] = fa;
+        if (two) exz[(sb * nz + j) * nx + i] = fb;
+    }
+}
+
 // (Ex, Ez) at the gather's four cells (i0 .. i0 + 1, j0 .. j0 + 1, with
 // 0 <= i0 <= nx - 2 and 0 <= j0 <= nz - 2) straight from phi: the 4 x 4
 // stencil block minus its corners, rows and columns clamped to the grid as
@@ -1216,7 +1278,7 @@ inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 // in cbuf), column FFT x Green's function x inverse column FFT, inverse row
 // FFTs unpacked to phi.  twt / green: the per-solve tables.
 void launch_fft_poisson(hipStream_t st, const ffa_sc_params &sp, const ffa_sc_result *res, const int64_t *grid,
-                        float2 *cbuf, float *phi, const float2 *twt, const float *green) {
+                        float2 *cbuf, float *phi, const float2 *twt, const float *green, bool inv_rows = true) {
     const bool p2 = is_pow2(sp.nx) && is_pow2(sp.nz);
     const fft_plan px = make_fft_plan(sp.nx), pz = make_fft_plan(sp.nz);
     const int lognx = ilog2(sp.nx), lognz = ilog2(sp.nz);
@@ -1234,8 +1296,9 @@ void launch_fft_poisson(hipStream_t st, const ffa_sc_params &sp, const ffa_sc_re
         else
             hipLaunchKernelGGL(k_fft_rows_fwd<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, sp, twt, px, rows);
         hipLaunchKernelGGL(k_fft_cols_green<true>, dim3(cblocks), dim3(BLOCK), cols_lds, st, cbuf, sp, twt, green, pz);
-        hipLaunchKernelGGL(k_fft_rows_inv<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
-                           lognx, lognz, rows);
+        if (inv_rows)
+            hipLaunchKernelGGL(k_fft_rows_inv<true>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
+                               lognx, lognz, rows);
     } else {
         if (grid)
             hipLaunchKernelGGL(k_fft_rows_fwd_grid<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, grid, cbuf, sp,
@@ -1244,8 +1307,9 @@ void launch_fft_poisson(hipStream_t st, const ffa_sc_params &sp, const ffa_sc_re
             hipLaunchKernelGGL(k_fft_rows_fwd<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, sp, twt, px,
                                rows);
         hipLaunchKernelGGL(k_fft_cols_green<false>, dim3(cblocks), dim3(BLOCK), cols_lds, st, cbuf, sp, twt, green, pz);
-        hipLaunchKernelGGL(k_fft_rows_inv<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
-                           lognx, lognz, rows);
+        if (inv_rows)
+            hipLaunchKernelGGL(k_fft_rows_inv<false>, dim3(rblocks), dim3(BLOCK), rows_lds, st, cbuf, phi, sp, twt, px,
+                               lognx, lognz, rows);
     }
 }
 
@@ -1341,12 +1405,27 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     else
         hipLaunchKernelGGL(k_sc_deposit_slices, dim3((unsigned)(sp->ns * nbands)), dim3(BLOCK), tile_lds, st, *sp, res,
                            rstart, rec2, reinterpret_cast<unsigned long long *>(grid), cell, bz);
-    launch_fft_poisson(st, *sp, res, pairs ? nullptr : grid, cbuf, phi, twt, green);
     // The gather from phi reads 12 potential values per particle instead of 4
     // field pairs: cheaper while the potential stays cache-resident (measured
-    // faster at 4.2M cells, slower at 33.5M), so larger grids store the cubes
+    // faster at 4.2M cells, slower at 33.5M), so larger grids store the cubes.
+    // With cubes the inverse row FFTs and the field stencils run as one
+    // kernel; phi is stored only when the caller asked for the cubes (a
+    // checking mode that keeps every intermediate).
     const bool cubes = sp->field_cubes != 0 || C > FROM_PHI_MAX_CELLS;
-    if (cubes) launch_field(st, *sp, res, phi, exz, nullptr, nullptr);
+    launch_fft_poisson(st, *sp, res, pairs ? nullptr : grid, cbuf, phi, twt, green, !cubes);
+    if (cubes) {
+        const fft_plan plx = make_fft_plan(sp->nx);
+        const int L = field_band_lines(sp->nx), R = L - 2;
+        const dim3 g((unsigned)(((sp->ns + 1) / 2) * ((sp->nz + R - 1) / R)));
+        const size_t lds_b = sizeof(float2) * ((size_t)2 * L * sp->nx + sp->nx);
+        float *phi_out = sp->field_cubes != 0 ? phi : nullptr;
+        if (is_pow2(sp->nx))
+            hipLaunchKernelGGL(k_fft_rows_inv_field<true>, g, dim3(BLOCK), lds_b, st, cbuf, phi_out, exz, *sp, res, twt,
+                               plx, ilog2(sp->nx));
+        else
+            hipLaunchKernelGGL(k_fft_rows_inv_field<false>, g, dim3(BLOCK), lds_b, st, cbuf, phi_out, exz, *sp, res,
+                               twt, plx, 0);
+    }
     if (p->n > 0) {
         const dim3 g((unsigned)((p->n + GATHER_U * BLOCK - 1) / (GATHER_U * BLOCK)));
         if (p->esc_f32) {

@@ -64,7 +64,9 @@ def test_spiral_cycle_full_size_properties(tmp_path):
         assert int(B.sc_view("grid").sum().item()) == alive << (2 * bits)
         assert int(B.sc_view("slice_count").sum().item()) == alive
         assert int(B.sc_view("cell_count").sum().item()) == alive
-        outs.append((B.esc_pre.clone(), B.sc_view("phi").clone()))     # the new endpoint field
+        # the new endpoint field and the field cubes (a grid above 2^23
+        # cells: the cubes are stored, phi stays in LDS)
+        outs.append((B.esc_pre.clone(), B.sc_view("exz").clone()))
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
     assert torch.isfinite(outs[0][1]).all() and outs[0][1].abs().amax().item() > 0.0
 
