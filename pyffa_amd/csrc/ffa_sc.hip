@@ -1,13 +1,13 @@
 // 2.5D space-charge solve of the tracking hot path (gfx950).
 //
 // Pipeline of one solve (every kernel on the context stream, no host sync):
-//   k_sc_stage1        per survivor: gamma, phi mod 2pi; per-block partial
-//                      sums of gamma and the four covering-arc min/max
+//   k_sc_front         per survivor: gamma, phi mod 2pi, SEO r0/pr0 and
+//                      x = r - r0; per-block partial sums of gamma, the four
+//                      covering-arc min/max and the moments
 //   k_sc_final1        1 block: covering arc, slice step, mean gamma,
 //                      perimeter, bunch length (device-side scalars); zeroes
 //                      the per-slice counters
-//   k_sc_locate_moments per survivor: SEO r0/pr0, x = r - r0, slice index,
-//                      per-block partial moments, slice histogram
+//   k_sc_slices        per survivor: slice index; slice histogram
 //   k_sc_final2        1 block: grid box, dx, dz, charge scale, slice offsets
 //   k_sc_bin           per survivor: CIC cell + fixed-point weights packed in
 //                      one 64-bit record, written slice-sorted (two-level
@@ -24,9 +24,11 @@
 //   k_fft_rows_fwd*    row FFTs of the packed slice pairs (reading and
 //                      scaling the int64 grid on small grids)
 //   k_fft_cols_green   column FFT, x periodic Green's function, inverse
-//   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi
-//   k_sc_field         -grad(phi)/gamma^2 with the reference's edge stencils,
-//                      stored as (Ex, Ez) cubes (only when sp.field_cubes)
+//   k_fft_rows_inv     inverse row FFTs, 1/(nx nz), unpack to phi (grids
+//                      whose gather differentiates phi itself)
+//   k_fft_rows_inv_field  inverse row FFTs fused with -grad(phi)/gamma^2 and
+//                      the reference's edge stencils, stored as (Ex, Ez)
+//                      cubes (above 2^23 cells, or when sp.field_cubes)
 //   k_sc_gather_roll   bilinear gather + pr0 projection, endpoint roll; without
 //                      the cubes it applies the same stencils to phi at the
 //                      particle's four cells
@@ -106,64 +108,6 @@ __device__ __forceinline__ int32_t wave_counter_add(int32_t *ctr, int key, bool 
     }
     if (todo) out = atomicAdd(&ctr[key], 1);
     return out;
-}
-
-// GetLocalCoordinates_2p5(_rk) + divide_arc2's scan (src/FFAG_SC_new.py:74-148, 534-566).
-__global__ __launch_bounds__(BLOCK) void k_sc_stage1(ffa_particles p, int rk, double c, double E0J,
-                                                    double *part) {
-    __shared__ double sh[10][BLOCK / 64];
-    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
-    double v[10] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30};
-    // UNR particles per pass with their loads issued together; a thread
-    // still visits its particles i, i + S, i + 2S, ... in that order
-    const int64_t S = (int64_t)NB * BLOCK;
-    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
-        // flags, then every state load, then the arithmetic: no branch between
-        // the particles' loads, so their round trips overlap
-        bool ok[UNR];
-        int64_t jj[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int64_t i = i0 + u * S;
-            const uint8_t f = p.flags[i < p.n ? i : 0];
-            ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
-            jj[u] = ok[u] ? i : 0;
-        }
-        double r[UNR], fd[UNR], vr[UNR], vz[UNR], fi[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            r[u] = p.r[jj[u]];
-            fd[u] = p.fidot[jj[u]];
-            vr[u] = p.vr[jj[u]];
-            vz[u] = p.vz[jj[u]];
-            fi[u] = p.fi[jj[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            if (!ok[u]) continue;
-            double g;                                  // gamma_of on the loaded state
-            if (rk) {
-                g = fd[u] / E0J;
-            } else {
-                const double rd = r[u] * fd[u];
-                const double v2 = vr[u] * vr[u] + vz[u] * vz[u] + rd * rd;
-                g = 1.0 / sqrt(1.0 - v2 / (c * c));
-            }
-            v[0] += g;
-            v[1] += 1.0;
-            const double fm = mod2pi(fi[u]);
-            for (int k = 0; k < 4; ++k) {
-                const double a = fm + shifts[k], b = 2.0 * FFA_PI;
-                const double q = a - b * floor(a / b);
-                v[2 + k] = q < v[2 + k] ? q : v[2 + k];
-                v[6 + k] = q > v[6 + k] ? q : v[6 + k];
-            }
-        }
-    }
-    const int op[10] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
-    block_reduce<10>(sh, v, op);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 10; ++k) part[16 * blockIdx.x + k] = v[k];
 }
 
 __device__ double py_mod(double a, double b) {  // Python float % (sign of divisor)
@@ -259,33 +203,29 @@ __device__ __forceinline__ double seo_bilinear(const double *Z, int NX, int NY, 
 // SEO lookup + ComputeSliceBoundsAndCoordsSBS_merged (src/FFAG_SC_new.py:449-531).
 constexpr int MAX_NS_LDS = 4096;
 
-// Particles per pass of k_sc_locate_moments: two keeps it under 128 VGPRs,
-// so all NB blocks are resident at once (the visiting order i0, i0 + S, ...
-// and hence the sums are the same for any value).
+// Particles per pass of k_sc_front: two keep it under 128 VGPRs, so all NB
+// blocks are resident at once (the visiting order i0, i0 + S, ... and hence
+// the sums are the same for any value).
 constexpr int LUNR = 2;
 
-// Per survivor: x = r - r0(Ek, phi), pr0 and the slice index (written for
-// the bin and gather kernels), the moments (sum x, z, x^2, z^2) and the
-// slice histogram: per-block counts in LDS, one global atomic per non-empty
-// (block, slice) pair (ns <= FFA_SC_MAX_NS).  Fixed block partition and
-// per-thread order, so the sums are reproducible run to run; LUNR particles
-// per pass have their loads (state, then the SEO table corners) in flight
-// together.
-__global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ffa_sc_params sp, double c,
-                                                            double E0J, const ffa_sc_result *res, double *px,
-                                                            double *ppr0, int32_t *pslice, int32_t *slc,
-                                                            int32_t *blk_off, double *part) {
-    __shared__ double sh[4][BLOCK / 64];
-    __shared__ int32_t hist[MAX_NS_LDS];
-    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
-    __syncthreads();
-    const double fs = res->fi_s, step = res->slice_step;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+// The per-particle front end of a solve in one pass over the state:
+// GetLocalCoordinates_2p5(_rk) and divide_arc2's scan
+// (src/FFAG_SC_new.py:74-148, 534-566: sum of gamma, survivor count, the
+// covering-arc minima / maxima of the azimuth under four shifts) and the SEO
+// lookup of ComputeSliceBoundsAndCoordsSBS_merged (x = r - r0(Ek, phi) and
+// pr0 written for the later kernels, the moments sum x, z, x^2, z^2).  None
+// of it needs the arc, which only the slice index does (k_sc_slices, after
+// k_sc_final1).  Fixed block partition and per-thread order: the sums are
+// reproducible run to run.  part[16 b + 0 .. 13] per block.
+__global__ __launch_bounds__(BLOCK) void k_sc_front(ffa_particles p, ffa_sc_params sp, double c, double E0J,
+                                                   double *px, double *ppr0, double *part) {
+    __shared__ double sh[14][BLOCK / 64];
+    const double shifts[4] = {0.0, FFA_PI, FFA_PI * 0.5, FFA_PI * 1.5};
+    double v[14] = {0.0, 0.0, 1e30, 1e30, 1e30, 1e30, -1e30, -1e30, -1e30, -1e30, 0.0, 0.0, 0.0, 0.0};
     const int64_t S = (int64_t)NB * BLOCK;
     for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += LUNR * S) {
-        // phases over the LUNR particles (flags; state; SEO corners; results),
-        // all loads ahead of the stores so the particles' memory round trips
-        // overlap
+        // flags, then every state load, then the arithmetic: no branch between
+        // the particles' loads, so their round trips overlap
         bool ok[LUNR];
         int64_t jj[LUNR];
 #pragma unroll
@@ -295,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
             jj[u] = ok[u] ? i : 0;
         }
-        double ek[LUNR], fm[LUNR], x[LUNR], z[LUNR], fd[LUNR], vr[LUNR], vz[LUNR];
+        double fm[LUNR], x[LUNR], z[LUNR], fd[LUNR], vr[LUNR], vz[LUNR];
 #pragma unroll
         for (int u = 0; u < LUNR; ++u) {
             x[u] = p.r[jj[u]];
@@ -305,6 +245,7 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             fm[u] = p.fi[jj[u]];
             z[u] = p.z[jj[u]];
         }
+        double ek[LUNR];
 #pragma unroll
         for (int u = 0; u < LUNR; ++u) {
             // gamma_of's arithmetic on the loaded state
@@ -318,6 +259,16 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             }
             ek[u] = (g - 1) * sp.E0_MeV;
             fm[u] = mod2pi(fm[u]);
+            if (ok[u]) {
+                v[0] += g;
+                v[1] += 1.0;
+                for (int k = 0; k < 4; ++k) {
+                    const double a = fm[u] + shifts[k], b = 2.0 * FFA_PI;
+                    const double q = a - b * floor(a / b);
+                    v[2 + k] = q < v[2 + k] ? q : v[2 + k];
+                    v[6 + k] = q > v[6 + k] ? q : v[6 + k];
+                }
+            }
         }
         double pr0[LUNR];
 #pragma unroll
@@ -325,41 +276,68 @@ __global__ __launch_bounds__(BLOCK) void k_sc_locate_moments(ffa_particles p, ff
             x[u] -= seo_bilinear(sp.seo_r, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
             pr0[u] = seo_bilinear(sp.seo_pr, sp.n_ek, sp.n_fi, sp.ek0, sp.dek, sp.fi0, sp.dfi, ek[u], fm[u]);
         }
-        int s[LUNR];
 #pragma unroll
         for (int u = 0; u < LUNR; ++u) {
-            double f = fm[u];
+            if (!ok[u]) continue;
+            const int64_t i = i0 + u * S;
+            px[i] = x[u];
+            ppr0[i] = pr0[u];
+            v[10] += x[u];
+            v[11] += z[u];
+            v[12] += x[u] * x[u];
+            v[13] += z[u] * z[u];
+        }
+    }
+    const int op[14] = {0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 0, 0, 0, 0};
+    block_reduce<14>(sh, v, op);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 14; ++k) part[16 * blockIdx.x + k] = v[k];
+}
+
+// Slice index of every survivor from the covering arc (k_sc_final1), and the
+// slice histogram: per-block counts in LDS, one global atomic per non-empty
+// (block, slice) pair (ns <= FFA_SC_MAX_NS).  Same block partition as
+// k_sc_bin, which places each block's records from these offsets.
+__global__ __launch_bounds__(BLOCK) void k_sc_slices(ffa_particles p, ffa_sc_params sp, const ffa_sc_result *res,
+                                                    int32_t *pslice, int32_t *slc, int32_t *blk_off) {
+    __shared__ int32_t hist[MAX_NS_LDS];
+    for (int k = threadIdx.x; k < sp.ns; k += BLOCK) hist[k] = 0;
+    __syncthreads();
+    const double fs = res->fi_s, step = res->slice_step;
+    const int64_t S = (int64_t)NB * BLOCK;
+    for (int64_t i0 = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i0 < p.n; i0 += UNR * S) {
+        bool ok[UNR];
+        double fi[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int64_t i = i0 + u * S;
+            const uint8_t f = p.flags[i < p.n ? i : 0];
+            ok[u] = (i < p.n) & ((f & FFA_F_ALIVE) != 0);
+            fi[u] = p.fi[ok[u] ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            double f = mod2pi(fi[u]);
             if (f < fs) f += FFA_TWO_PI;
             double fsl = (f - fs) / step;
             fsl = fsl > -1e9 ? (fsl < 1e9 ? fsl : 1e9) : -1e9;
             int64_t sl = (int64_t)fsl;
             if (sl < 0) sl = 0; else if (sl >= sp.ns) sl = sp.ns - 1;
-            s[u] = ok[u] ? (int)sl : -1;
+            const int s = ok[u] ? (int)sl : -1;
             const int64_t i = i0 + u * S;
-            if (i < p.n) {
-                pslice[i] = s[u];
-                if (ok[u]) { px[i] = x[u]; ppr0[i] = pr0[u]; }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < LUNR; ++u) {
-            if (s[u] >= 0) { v[0] += x[u]; v[1] += z[u]; v[2] += x[u] * x[u]; v[3] += z[u] * z[u]; }
-            wave_counter_add(hist, s[u] >= 0 ? s[u] : 0, s[u] >= 0);
+            if (i < p.n) pslice[i] = s;
+            wave_counter_add(hist, s >= 0 ? s : 0, s >= 0);
         }
     }
-    const int op[4] = {0, 0, 0, 0};
-    block_reduce<4>(sh, v, op);       // contains __syncthreads: hist is complete
+    __syncthreads();
     // the slice counter's value before this block's add is where the block's
     // run of records starts inside the slice (entries of slices the block
     // does not hold stay unwritten and are never read)
     int32_t *bo = blk_off + (size_t)blockIdx.x * sp.ns;
     for (int k = threadIdx.x; k < sp.ns; k += BLOCK)
         if (hist[k]) bo[k] = atomicAdd(&slc[k], hist[k]);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 4; ++k) part[16 * blockIdx.x + 10 + k] = v[k];
 }
 
-// Grid box from the moments (src/FFAG_Bunch.py:1259-1322).
 __global__ __launch_bounds__(BLOCK) void k_sc_final2(const double *part, ffa_sc_params sp, ffa_sc_result *res,
                                                     const int32_t *slc, int32_t *slice_start) {
     __shared__ double sh[4][BLOCK / 64];
@@ -432,7 +410,7 @@ __device__ __forceinline__ uint64_t cic_record(double x, double z, const ffa_sc_
 }
 
 // Bin the survivors into slice-sorted records.  Same block partition and
-// per-thread order as k_sc_locate_moments, which counted each block's
+// per-thread order as k_sc_slices, which counted each block's
 // particles per slice and reserved the block's run inside each slice: the
 // LDS counters start at the runs' global offsets and each record takes the
 // next place of its slice's run.
@@ -1377,10 +1355,9 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
     uint64_t *rec = reinterpret_cast<uint64_t *>(w + L->records);
     int32_t *sstart = reinterpret_cast<int32_t *>(w + L->slice_start);
     int32_t *blk_off = reinterpret_cast<int32_t *>(w + L->block_offset);
-    hipLaunchKernelGGL(k_sc_stage1, dim3(NB), dim3(BLOCK), 0, st, *p, sp->rk, sp->c, E0J, part);
+    hipLaunchKernelGGL(k_sc_front, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, px, ppr0, part);
     hipLaunchKernelGGL(k_sc_final1, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc);
-    hipLaunchKernelGGL(k_sc_locate_moments, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, sp->c, E0J, res, px, ppr0, pslice,
-                       slc, blk_off, part);
+    hipLaunchKernelGGL(k_sc_slices, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, pslice, slc, blk_off);
     hipLaunchKernelGGL(k_sc_final2, dim3(1), dim3(BLOCK), 0, st, part, *sp, res, slc, sstart);
     hipLaunchKernelGGL(k_sc_bin, dim3(NB), dim3(BLOCK), 0, st, *p, *sp, res, px, pslice, sstart, blk_off, rec);
     // Small grids deposit slice by slice into the int64 grid, which the row
