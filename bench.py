@@ -190,13 +190,14 @@ def cpu_baseline(M, gp, n_sample, blocks, n_total):
     workload: its first ``n_sample`` particles (macro-particle charge of the
     full bunch) for ``blocks`` x SC_STEP Boris steps with one SC solve per
     SC_STEP steps on the workload's grid."""
+    from oracle import adapters as A
     from oracle import oracle as O
-    from tests import cases
+    from pyffa_amd import workloads as W
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     O.lib().ora_set_threads(threads)
     S = M[:n_sample].copy()
-    tr = O.Tracker(cases.oracle_machine(gp), S, enable_sc=True, sc_step=SC_STEP, grid=GRID, marcosize=1e13 / n_total,
-                   seo=cases.oracle_seo())
+    tr = O.Tracker(A.machine_from(gp), S, enable_sc=True, sc_step=SC_STEP, grid=GRID, marcosize=1e13 / n_total,
+                   seo=A.seo_from(W.seo("stable")))
     t0 = time.perf_counter()
     tr.run(0.0, DT, blocks * SC_STEP)
     el = time.perf_counter() - t0
