@@ -13,10 +13,12 @@ reduction and, in the tiers that order particles, one azimuth sort.
 
 The headline value is the FP32-field tier (FP64 state, fused steps).  Beside
 it, on the same workload: the FP64 map-gather path in the reference's
-particle order (the parity path) and the FP64 path with the same azimuth
-ordering as the FP32 tier, so the speed-ups separate what FP32 buys from
-what ordering buys; the FP32 tier's accuracy against FP64 (centroid, rms
-emittance) and its field error against the FP64 field at sampled points.
+particle order (the parity path), the FP64 path with the same azimuth
+ordering as the FP32 tier, and that ordered FP64 path with its steps fused
+into one launch per block (bit-identical to it), so the speed-ups separate
+what FP32 buys from what ordering and fusion buy; the FP32 tier's accuracy
+against FP64 (centroid, rms emittance) and its field error against the FP64
+field at sampled points.
 
 Further legs (sub-objects of the JSON line):
 * ``surrogate`` -- the field network trained on the GPU on 4M samples of the
@@ -113,7 +115,8 @@ def accuracy(Sa, Sb, steps):
             "compared": ma["alive"], "after_steps": steps}
 
 
-def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_every=None, marcosize=None):
+def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_every=None, marcosize=None,
+             fuse_fp64=False):
     """Tracks the bunch ``warmup`` untimed then ``steps`` timed bench steps
     (PUSHES push steps each) through the tracking loop; returns the bunch,
     the tracker, the timed seconds, the final state in input order and the
@@ -136,6 +139,8 @@ def run_tier(precision, M, gp, warmup, steps, grid=GRID, surrogate=None, sort_ev
             d["surrogate"] = surrogate
         if sort_every is not None:
             d["sort_every"] = sort_every
+        if fuse_fp64:
+            d["fuse_fp64"] = True
         if timer is not None:
             d["kernel_timer"] = timer
         return d
@@ -632,11 +637,20 @@ def main():
         B64, rk64, el64, S64, k64 = run_tier("fp64", M, gp, args.warmup, k)
         del B64, rk64
         log(f"fp64 path (input order) {el64:.3f} s, kernels {k64}")
-        B64s, rk64s, el64s, _, k64s = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES)
+        B64s, rk64s, el64s, S64s, k64s = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES)
         del B64s, rk64s
         log(f"fp64 path (azimuth-ordered) {el64s:.3f} s, kernels {k64s}")
+        B64f, rk64f, el64f, S64f, k64f = run_tier("fp64", M, gp, args.warmup, k, sort_every=PUSHES, fuse_fp64=True)
+        del B64f, rk64f
+        log(f"fp64 path (azimuth-ordered, steps fused) {el64f:.3f} s, kernels {k64f}")
+        same = all(bool((S64s[key] == S64f[key]).all()) for key in S64s)
+        log(f"fp64 fused steps bit-identical to one launch per step: {same}")
         out["fp64_value"] = ps64 / el64
         out["fp64_sorted_value"] = ps64 / el64s
+        out["fp64_fused_sorted_value"] = ps64 / el64f
+        out["speedup_fp32_vs_fp64_fused_sorted"] = (ps / el32) / (ps64 / el64f)
+        out["fp64_fused_sorted_kernel_ms"] = k64f
+        out["fp64_fused_bit_identical"] = same
         out["speedup_fp32_vs_fp64"] = (ps / el32) / (ps64 / el64)
         out["speedup_fp32_vs_fp64_sorted"] = (ps / el32) / (ps64 / el64s)
         out["fp64_kernel_ms"] = k64

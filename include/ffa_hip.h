@@ -219,6 +219,15 @@ int ffa_push_rk4_f64(void *ctx, const ffa_particles *p, const ffa_step *s);
 int ffa_push_f64_multi(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
                        int32_t nsteps, int32_t rk4);
 
+/* The same nsteps Boris steps as ffa_push_f64_multi(rk4 = 0) -- the loop body
+ * of src/FFAG_track.py:331-556 for steps without an SC solve, each step the
+ * one ffa_push_boris_f64 replaces -- in one launch with the state held in
+ * registers between steps: bit-identical results without the state's round
+ * trip through HBM at every step (nsteps <= 2978, the context's step table).  Host arrays as for
+ * ffa_push_f64_multi. */
+int ffa_push_boris_f64_fused(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                             int32_t nsteps);
+
 /* nsteps consecutive Boris steps of the FP32-field tier in one launch
  * (state and RF phase in FP64, field evaluation and SC kick in FP32, state
  * held in registers between steps).  steps: host array of nsteps entries.

@@ -17,6 +17,9 @@ between two SC solves as one fused launch), ``verbose`` (progress lines),
 ``start_step`` (continue a run: the step index the loop starts from, so the
 SC schedule and endpoint history carry on; ``max_stepsN`` stays the absolute
 end), ``max_block`` (cap on the steps issued per launch block) and
+``fuse_fp64`` (FP64 Boris path: run each block of steps as one launch with
+the state in registers, ``ffa_push_boris_f64_fused``; bit-identical to the
+default one launch per step) and
 ``kernel_timer`` (an object whose ``begin(name)`` / ``end(name, launches)``
 the loop calls around each push call, name ``"push"``, with the number of
 kernel launches it issued (1 for a fused FP32 block, one per step on the FP64
@@ -154,6 +157,7 @@ class FFAG_RungeKutta:
         steps_i = int(LocalParameters.get("start_step", 0))
         max_block = int(LocalParameters.get("max_block", _MAX_BLOCK))
         fused = precision == "fp32" and solver == "boris"
+        fuse64 = solver == "boris" and not fused and bool(LocalParameters.get("fuse_fp64", False))
         timer = LocalParameters.get("kernel_timer")
         n_solves = 0
         while steps_i < max_steps:
@@ -224,11 +228,14 @@ class FFAG_RungeKutta:
             if fused:
                 ctx.check(lib.ffa_push_boris_fused_f32(ctx.h, BunchObj.particles(), steps, inj, nb),
                           "ffa_push_boris_fused_f32")
+            elif fuse64:
+                ctx.check(lib.ffa_push_boris_f64_fused(ctx.h, BunchObj.particles(), steps, inj, nb),
+                          "ffa_push_boris_f64_fused")
             else:
                 ctx.check(lib.ffa_push_f64_multi(ctx.h, BunchObj.particles(), steps, inj, nb,
                                                  1 if solver == "rk4" else 0), "ffa_push_f64_multi")
             if timer is not None:
-                timer.end("push", 1 if fused else nb)
+                timer.end("push", 1 if fused or fuse64 else nb)
             BunchObj.last_sc_s = steps[nb - 1].sc_s
             BunchObj.UpdatePostSteps()
             if examine:

@@ -251,6 +251,30 @@ int ffa_push_f64_multi(void *ctx, const ffa_particles *p, const ffa_step *steps,
     return FFA_OK;
 }
 
+int ffa_push_boris_f64_fused(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                             int32_t nsteps) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !steps || !inj) return fail(c, FFA_ERR_INVALID, "ctx, steps or inj is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP64 push needs a double Taylor map");
+    if (int rc = check_particles(c, p)) return rc;
+    if (p->esc_f32) return fail(c, FFA_ERR_INVALID, "the FP64 push needs double SC endpoints");
+    if (nsteps < 0 || nsteps > FFA_MAX_FUSED || (size_t)nsteps * 16 > FFA_CTX_INJ_BYTES)
+        return fail(c, FFA_ERR_INVALID, "nsteps %d outside [0, %d]", nsteps, (int)(FFA_CTX_INJ_BYTES / 16));
+    for (int k = 0; k < nsteps; ++k)
+        if (inj[2 * k] < 0 || inj[2 * k + 1] > p->n || inj[2 * k] > inj[2 * k + 1])
+            return fail(c, FFA_ERR_INVALID, "inject range of step %d invalid", k);
+    if (nsteps == 0) return FFA_OK;
+    hipError_t e = hipMemcpyAsync(c->dev + FFA_CTX_STEPS_OFF, steps, sizeof(ffa_step) * nsteps,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dev + FFA_CTX_INJ_OFF, inj, sizeof(int64_t) * 2 * nsteps, hipMemcpyHostToDevice,
+                           c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "ffa_push_boris_f64_fused (step table)");
+    return hip_check(c, launch_boris_f64_multi(c, p, nsteps), "ffa_push_boris_f64_fused");
+}
+
 int ffa_push_boris_fused_f32(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
                              int32_t nsteps) {
     ffa_ctx_s *c = as_ctx(ctx);

@@ -105,6 +105,7 @@ hipError_t launch_inject(ffa_ctx_s *c, const ffa_particles *p, int64_t b, int64_
 hipError_t launch_boris_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s);
 hipError_t launch_rk4_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s);
 hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps);
+hipError_t launch_boris_f64_multi(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps);
 hipError_t launch_reduce_phi(ffa_ctx_s *c, const ffa_particles *p, double *out_dev);
 hipError_t launch_bfield_eval(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
                               double *out, int mode, double zt);

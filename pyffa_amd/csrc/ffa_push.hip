@@ -211,23 +211,20 @@ __global__ __launch_bounds__(BLOCK) void k_inject(uint8_t *flags, int64_t b, int
     if (i < e) flags[i] = (uint8_t)(flags[i] | FFA_F_INJ | FFA_F_ALIVE);
 }
 
-// One FP64 Boris step: A2 extrapolation + A6..A15.
-__global__ __launch_bounds__(BLOCK) void k_boris_f64(ffa_particles p, const ffa_machine *__restrict__ mp,
-                                                    ffa_step s) {
-    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-    if (i >= p.n) return;
-    const ffa_machine &m = *mp;
-    const int64_t n = p.n;
-    const double r = p.r[i], vr = p.vr[i], z = p.z[i], vz = p.vz[i], fi = p.fi[i], fidot = p.fidot[i];
-    const double rf = p.rf_phase[i];
-    uint8_t fl = p.flags[i];
-    const double *ep = reinterpret_cast<const double *>(p.esc_pre);
-    const double *epp = reinterpret_cast<const double *>(p.esc_prepre);
-    double esr = ep[i], esz = ep[n + i], esf = ep[2 * n + i];
+// One FP64 Boris step of one particle: A2 extrapolation + A6..A15, on the
+// cylindrical state in place (every particle, live or not, goes through the
+// reference's cylindrical -> Cartesian -> cylindrical round trip).  Shared
+// by the one-step kernel and the multi-step kernel, so both give the same
+// bits (this unit keeps the operation order: no contraction).
+// e_pre / e_prepre: the particle's SC endpoint fields (r, z, fi).
+__device__ __forceinline__ void boris_step_f64(const ffa_machine &m, const ffa_step &s, double &r, double &vr,
+                                               double &z, double &vz, double &fi, double &fidot, double &rf,
+                                               uint8_t &fl, const double e_pre[3], const double e_prepre[3]) {
+    double esr = e_pre[0], esz = e_pre[1], esf = e_pre[2];
     if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) {
-        esr = esr + s.sc_s * (esr - epp[i]);
-        esz = esz + s.sc_s * (esz - epp[n + i]);
-        esf = esf + s.sc_s * (esf - epp[2 * n + i]);
+        esr = esr + s.sc_s * (esr - e_prepre[0]);
+        esz = esz + s.sc_s * (esz - e_prepre[1]);
+        esf = esf + s.sc_s * (esf - e_prepre[2]);
     }
     const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
     const double sinFi = sin(fi), cosFi = cos(fi);
@@ -284,14 +281,75 @@ __global__ __launch_bounds__(BLOCK) void k_boris_f64(ffa_particles p, const ffa_
     sincos(fi_new, &sn, &cn);
     const double vr_n = vxn * cn + vyn * sn;
     const double vfi = -vxn * sn + vyn * cn;
-    p.r[i] = r_new;
-    p.vr[i] = vr_n;
-    p.z[i] = zn;
-    p.vz[i] = vzn;
-    p.fi[i] = fi_new;
-    p.fidot[i] = r_new > 1e-12 ? vfi / r_new : 0.0;
-    p.rf_phase[i] = rf + s.dRF;
-    if (fl != p.flags[i]) p.flags[i] = fl;
+    r = r_new;
+    vr = vr_n;
+    z = zn;
+    vz = vzn;
+    fi = fi_new;
+    fidot = r_new > 1e-12 ? vfi / r_new : 0.0;
+    rf = rf + s.dRF;
+}
+
+// One FP64 Boris step of every particle.
+__global__ __launch_bounds__(BLOCK) void k_boris_f64(ffa_particles p, const ffa_machine *__restrict__ mp,
+                                                    ffa_step s) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    double r = p.r[i], vr = p.vr[i], z = p.z[i], vz = p.vz[i], fi = p.fi[i], fidot = p.fidot[i];
+    double rf = p.rf_phase[i];
+    const uint8_t fl0 = p.flags[i];
+    uint8_t fl = fl0;
+    const double *ep = reinterpret_cast<const double *>(p.esc_pre);
+    const double *epp = reinterpret_cast<const double *>(p.esc_prepre);
+    const double e0[3] = {ep[i], ep[n + i], ep[2 * n + i]};
+    double e1[3] = {0.0, 0.0, 0.0};
+    if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) { e1[0] = epp[i]; e1[1] = epp[n + i]; e1[2] = epp[2 * n + i]; }
+    boris_step_f64(*mp, s, r, vr, z, vz, fi, fidot, rf, fl, e0, e1);
+    p.r[i] = r;
+    p.vr[i] = vr;
+    p.z[i] = z;
+    p.vz[i] = vz;
+    p.fi[i] = fi;
+    p.fidot[i] = fidot;
+    p.rf_phase[i] = rf;
+    if (fl != fl0) p.flags[i] = fl;
+}
+
+// nsteps FP64 Boris steps of every particle in one launch, the state held in
+// registers between steps: the same steps as nsteps launches of k_boris_f64
+// (with the injection ranges inj[2 k], inj[2 k + 1] of step k applied first,
+// as ffa_inject does), bit for bit, without the state's round trip through
+// HBM at every step.  steps / inj: the context's device step table.
+__global__ __launch_bounds__(BLOCK) void k_boris_f64_multi(ffa_particles p, const ffa_machine *__restrict__ mp,
+                                                          const ffa_step *__restrict__ steps,
+                                                          const int64_t *__restrict__ inj, int nsteps) {
+    const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t n = p.n;
+    double r = p.r[i], vr = p.vr[i], z = p.z[i], vz = p.vz[i], fi = p.fi[i], fidot = p.fidot[i];
+    double rf = p.rf_phase[i];
+    const uint8_t fl0 = p.flags[i];
+    uint8_t fl = fl0;
+    const double *ep = reinterpret_cast<const double *>(p.esc_pre);
+    const double *epp = reinterpret_cast<const double *>(p.esc_prepre);
+    const double e0[3] = {ep[i], ep[n + i], ep[2 * n + i]};
+    double e1[3] = {0.0, 0.0, 0.0};
+    // the SC flags do not change inside a block of steps
+    if ((fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE)) { e1[0] = epp[i]; e1[1] = epp[n + i]; e1[2] = epp[2 * n + i]; }
+    const ffa_machine &m = *mp;
+    for (int k = 0; k < nsteps; ++k) {
+        if (i >= inj[2 * k] && i < inj[2 * k + 1]) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
+        boris_step_f64(m, steps[k], r, vr, z, vz, fi, fidot, rf, fl, e0, e1);
+    }
+    p.r[i] = r;
+    p.vr[i] = vr;
+    p.z[i] = z;
+    p.vz[i] = vz;
+    p.fi[i] = fi;
+    p.fidot[i] = fidot;
+    p.rf_phase[i] = rf;
+    if (fl != fl0) p.flags[i] = fl;
 }
 
 // RK4_equations_of_motion_njit (src/FFAG_MathTools.py:2205-2266) for one live
@@ -495,6 +553,15 @@ hipError_t launch_inject(ffa_ctx_s *c, const ffa_particles *p, int64_t b, int64_
 hipError_t launch_boris_f64(ffa_ctx_s *c, const ffa_particles *p, const ffa_step *s) {
     if (p->n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_boris_f64, dim3(grid_for(p->n)), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c), *s);
+    return hipGetLastError();
+}
+
+hipError_t launch_boris_f64_multi(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps) {
+    if (p->n == 0 || nsteps == 0) return hipSuccess;
+    const ffa_step *steps = reinterpret_cast<const ffa_step *>(c->dev + FFA_CTX_STEPS_OFF);
+    const int64_t *inj = reinterpret_cast<const int64_t *>(c->dev + FFA_CTX_INJ_OFF);
+    hipLaunchKernelGGL(k_boris_f64_multi, dim3(grid_for(p->n)), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
+                       steps, inj, nsteps);
     return hipGetLastError();
 }
 
