@@ -176,29 +176,31 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
         int gap = 0;
         const float acc = k->acc * v0;
         if (k->gap_model == 1) spiral_gap_f32(k, r, th, z, acc, Er, Efi, Ez, gap);
-        else for (int g = 0; g < k->n_gap; ++g) {
-            if (xf < k->gap_box[g][0] || xf > k->gap_box[g][1] || yf < k->gap_box[g][2] || yf > k->gap_box[g][3]) {
-                Er = 0.0f; Efi = 0.0f;
-                continue;
-            }
-            bool inside = true;
-            float prev = 0.0f;
-            for (int q = 0; q < 4; ++q) {
-                const float x0 = k->gap_px[g][q], y0 = k->gap_py[g][q];
-                const float x1 = k->gap_px[g][(q + 1) & 3], y1 = k->gap_py[g][(q + 1) & 3];
-                const float cross = (x1 - x0) * (yf - y0) - (y1 - y0) * (xf - x0);
-                if (q == 0) prev = cross;
-                else if (cross * prev < 0.0f) { inside = false; break; }
-            }
-            if (inside) {
-                float sd, cd;
-                __sincosf(k->gap_thn[g] - th, &sd, &cd);
-                Er = acc * cd;
-                Efi = acc * sd;
-                gap = g;
-            } else {
-                Er = 0.0f; Efi = 0.0f;
-                gap = 0;
+        else if (k->n_gap > 0) {
+            // The reference evaluates the gaps in order and each one sets or
+            // zeroes the field (src/FFAG_Field.py:296-306), so the last gap
+            // alone decides it: a particle outside the last gap's box or
+            // polygon gets E = 0 whatever the earlier gaps gave, and the gap
+            // index only matters where E != 0.  Evaluate that gap only.
+            const int g = k->n_gap - 1;
+            if (!(xf < k->gap_box[g][0] || xf > k->gap_box[g][1] || yf < k->gap_box[g][2] ||
+                  yf > k->gap_box[g][3])) {
+                bool inside = true;
+                float prev = 0.0f;
+                for (int q = 0; q < 4; ++q) {
+                    const float x0 = k->gap_px[g][q], y0 = k->gap_py[g][q];
+                    const float x1 = k->gap_px[g][(q + 1) & 3], y1 = k->gap_py[g][(q + 1) & 3];
+                    const float cross = (x1 - x0) * (yf - y0) - (y1 - y0) * (xf - x0);
+                    if (q == 0) prev = cross;
+                    else if (cross * prev < 0.0f) { inside = false; break; }
+                }
+                if (inside) {
+                    float sd, cd;
+                    __sincosf(k->gap_thn[g] - th, &sd, &cd);
+                    Er = acc * cd;
+                    Efi = acc * sd;
+                    gap = g;
+                }
             }
         }
         // the RF factor only matters inside a gap (E = 0 elsewhere)
@@ -270,15 +272,12 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     bool moved = false;
     const double q_over_m = q / mm;
     const bool aperture = m.aperture_enable != 0;
-    // first particle of this wave (wave-uniform): the injection test runs
-    // per lane only in waves that overlap the step's injection range
-    const int64_t wave0 = (int64_t)xcd_block() * (SUR ? SUR_BLOCK : BLOCK) +
-                          __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     const double ap0 = m.aperture[0], ap1 = m.aperture[1], ap2 = m.aperture[2], ap3 = m.aperture[3];
     for (int kstep = 0; kstep < nsteps; ++kstep) {
         const ffa_step &s = steps[kstep];
         const int64_t ilo = inj[2 * kstep], ihi = inj[2 * kstep + 1];
-        if (ilo < wave0 + 64 && ihi > wave0 && i >= ilo && i < ihi) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
+        // ilo <= i < ihi as one unsigned compare (the host checks ilo <= ihi)
+        if ((uint64_t)(i - ilo) < (uint64_t)(ihi - ilo)) fl = (uint8_t)(fl | FFA_F_INJ | FFA_F_ALIVE);
         const bool live = (fl & FFA_F_INJ) && (fl & FFA_F_ALIVE);
         bool use_sur = false;
         float sBz = 0.0f, sBfi = 0.0f, sBr = 0.0f;
