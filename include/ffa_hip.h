@@ -252,6 +252,27 @@ int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_s
  * ffa_sc_workspace_layout. */
 int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace);
 
+/* ffa_sc_solve up to the field, with its gather and endpoint roll left to
+ * the next ffa_push_boris_fused_f32_sc: the particles' endpoint fields and
+ * flags are not touched and the endpoint pointers are not swapped yet.  The
+ * tracking loop of the FP32 tier solves this way, so that the kicks are
+ * gathered inside the push launch that uses them (one pass over the
+ * particles fewer per solve).  Same checks as ffa_sc_solve.  This split has
+ * no counterpart in the reference, whose solve and push are one Python loop
+ * (src/FFAG_track.py:356-382, 449-478). */
+int ffa_sc_solve_field(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace);
+
+/* The gather and endpoint roll of the solve ffa_sc_solve_field left in
+ * workspace (same p, sp and workspace), then nsteps steps as
+ * ffa_push_boris_fused_f32 (nsteps may be 0: the gather alone).  Results
+ * are bit-identical to ffa_sc_solve followed by the pointer swap and
+ * ffa_push_boris_fused_f32: the gather runs per particle with the solve's
+ * own FP64 arithmetic.  p carries the endpoint pointers the solve saw; the
+ * caller swaps esc_pre and esc_prepre after this call, as after
+ * ffa_sc_solve.  Refused while the surrogate is enabled. */
+int ffa_push_boris_fused_f32_sc(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                                int32_t nsteps, const ffa_sc_params *sp, const void *workspace);
+
 /* ---- Stage-level space charge: the module functions of FFAG_SC_new -------
  * Each entry point below runs one stage of the 2.5D solve on device arrays
  * laid out as the reference's numpy arrays (the (N, 16) row-major particle

@@ -106,6 +106,7 @@ class FFAG_Bunch:
         self._seo_dev = None
         self._sc_ws = None
         self._sc_layout = None
+        self._pending_gather = None     # SC params of a solve whose gather the next push applies
         self._host_cache = None
         # store per-cell CIC base counts at each solve (a checking aid)
         self.sc_cell_counts = False
@@ -158,6 +159,8 @@ class FFAG_Bunch:
         """Reorder the injected particles by azimuth bin (stable; see
         ffa_sort_particles).  The host mirror and ``ordered()`` keep
         presenting particles in the input order."""
+        if self._pending_gather is not None:
+            raise RuntimeError("an SC solve's gather is pending: push before reordering the particles")
         count = self._uninject_head
         if count < 2:
             return
@@ -198,6 +201,8 @@ class FFAG_Bunch:
         """(k, 16) host matrix of the particles held at device positions
         ``pos`` (an int64 device tensor), or of all particles in the input
         order when ``pos`` is None."""
+        if self._pending_gather is not None:
+            raise RuntimeError("an SC solve's gather is pending: the endpoint fields are not written yet")
         t = self.torch
         if pos is None:
             take = self.ordered
@@ -522,16 +527,43 @@ class FFAG_Bunch:
                 res["bunch_length"], ef, s_grid, rho_f, M[alive, 0], M[alive, 4], px,
                 M[alive, 0] - px, M[alive, 2], fi_grid)
 
-    def sc_solve_device(self, ctx=None, *, first, s_last):
+    def sc_solve_device(self, ctx=None, *, first, s_last, defer_gather=False):
+        """One SC solve on the device.  defer_gather: stop after the field;
+        the gather and endpoint roll then run inside the next FP32 push
+        launch (``push_fused_f32``), which must follow before anything reads
+        the endpoint fields or flags."""
         if ctx is None:
             ctx = self._own_ctx()
+        if self._pending_gather is not None:
+            raise RuntimeError("an SC solve's gather is still pending: push before the next solve")
         ws, _ = self.sc_workspace()
         sp = self.sc_params(first=first, s_last=s_last)
+        if defer_gather:
+            ctx.check(ctx.lib.ffa_sc_solve_field(ctx.h, self.particles(), sp, ws.data_ptr()), "ffa_sc_solve_field")
+            self._pending_gather = sp
+            return
         ctx.check(ctx.lib.ffa_sc_solve(ctx.h, self.particles(), sp, ws.data_ptr()), "ffa_sc_solve")
+        self._finish_sc_roll()
+
+    def _finish_sc_roll(self):
         # the solve writes the new pre-step field into the prepre buffer and
         # leaves the old pre-step field, now the prepre one, where it was
         self.esc_pre, self.esc_prepre = self.esc_prepre, self.esc_pre
         self._host_cache = None
+
+    def push_fused_f32(self, ctx, steps, inj, nsteps):
+        """nsteps fused FP32 steps (ffa_push_boris_fused_f32), applying first
+        the gather of a solve that deferred it (ffa_push_boris_fused_f32_sc)."""
+        sp = self._pending_gather
+        if sp is None:
+            ctx.check(ctx.lib.ffa_push_boris_fused_f32(ctx.h, self.particles(), steps, inj, nsteps),
+                      "ffa_push_boris_fused_f32")
+            return
+        ws, _ = self.sc_workspace()
+        ctx.check(ctx.lib.ffa_push_boris_fused_f32_sc(ctx.h, self.particles(), steps, inj, nsteps, sp,
+                                                      ws.data_ptr()), "ffa_push_boris_fused_f32_sc")
+        self._pending_gather = None
+        self._finish_sc_roll()
 
 
 def ctypes_len(ty):

@@ -19,7 +19,10 @@ SC schedule and endpoint history carry on; ``max_stepsN`` stays the absolute
 end), ``max_block`` (cap on the steps issued per launch block) and
 ``fuse_fp64`` (FP64 Boris path: run each block of steps as one launch with
 the state in registers, ``ffa_push_boris_f64_fused``; bit-identical to the
-default one launch per step) and
+default one launch per step), ``defer_sc_gather`` (FP32 tier without the
+surrogate, default True: each SC solve stops after the field and the push
+launch that follows gathers the kicks, ``ffa_sc_solve_field`` +
+``ffa_push_boris_fused_f32_sc``; bit-identical to the separate gather) and
 ``kernel_timer`` (an object whose ``begin(name)`` / ``end(name, launches)``
 the loop calls around each push call, name ``"push"``, with the number of
 kernel launches it issued (1 for a fused FP32 block, one per step on the FP64
@@ -158,6 +161,11 @@ class FFAG_RungeKutta:
         max_block = int(LocalParameters.get("max_block", _MAX_BLOCK))
         fused = precision == "fp32" and solver == "boris"
         fuse64 = solver == "boris" and not fused and bool(LocalParameters.get("fuse_fp64", False))
+        # FP32 tier without the surrogate: each solve stops after the field
+        # and its gather runs inside the push launch that follows
+        # (bit-identical; one pass over the particles fewer per solve)
+        defer_gather = (fused and LocalParameters.get("surrogate") is None
+                        and bool(LocalParameters.get("defer_sc_gather", True)))
         timer = LocalParameters.get("kernel_timer")
         n_solves = 0
         while steps_i < max_steps:
@@ -194,7 +202,7 @@ class FFAG_RungeKutta:
                 s_last = ((steps_i - 1) % sc_step) / sc_step if steps_i - 1 >= sc_step else 0.0
                 if timer is not None:
                     timer.begin("sc")
-                BunchObj.sc_solve_device(ctx, first=(steps_i == 0), s_last=s_last)
+                BunchObj.sc_solve_device(ctx, first=(steps_i == 0), s_last=s_last, defer_gather=defer_gather)
                 if timer is not None:
                     timer.end("sc", 1)
                 n_solves += 1
@@ -226,8 +234,7 @@ class FFAG_RungeKutta:
             if timer is not None:
                 timer.begin("push")
             if fused:
-                ctx.check(lib.ffa_push_boris_fused_f32(ctx.h, BunchObj.particles(), steps, inj, nb),
-                          "ffa_push_boris_fused_f32")
+                BunchObj.push_fused_f32(ctx, steps, inj, nb)
             elif fuse64:
                 ctx.check(lib.ffa_push_boris_f64_fused(ctx.h, BunchObj.particles(), steps, inj, nb),
                           "ffa_push_boris_f64_fused")

@@ -142,7 +142,8 @@ _lib = None
 EXPORTED = ("ffa_create", "ffa_destroy", "ffa_last_error", "ffa_set_stream", "ffa_set_machine",
             "ffa_inject", "ffa_push_boris_f64", "ffa_push_rk4_f64", "ffa_push_f64_multi",
             "ffa_push_boris_f64_fused", "ffa_push_boris_fused_f32",
-            "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_reduce_phi", "ffa_bfield_eval",
+            "ffa_sc_workspace_layout", "ffa_sc_solve", "ffa_sc_solve_field", "ffa_push_boris_fused_f32_sc",
+            "ffa_reduce_phi", "ffa_bfield_eval",
             "ffa_mlp_workspace_layout", "ffa_mlp_pack", "ffa_mlp_grad", "ffa_mlp_adam", "ffa_mlp_train_step",
             "ffa_mlp_eval", "ffa_mlp_set", "ffa_sort_workspace_size", "ffa_sort_particles", "ffa_detect_crossing",
             "ffa_taylor_maps", "ffa_orbit_track", "ffa_spiral_gap_maps",
@@ -177,6 +178,9 @@ def load():
                                              ctypes.POINTER(_i64), _i32]
     lib.ffa_sc_workspace_layout.argtypes = [_i64, _i32, _i32, _i32, ctypes.POINTER(FfaScLayout)]
     lib.ffa_sc_solve.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams), _vp]
+    lib.ffa_sc_solve_field.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams), _vp]
+    lib.ffa_push_boris_fused_f32_sc.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaStep),
+                                                ctypes.POINTER(_i64), _i32, ctypes.POINTER(FfaScParams), _vp]
     lib.ffa_sc_solve_noalloc.argtypes = [_vp, ctypes.POINTER(FfaParticles), ctypes.POINTER(FfaScParams)]
     lib.ffa_reduce_phi.argtypes = [_vp, ctypes.POINTER(FfaParticles), _vp]
     lib.ffa_bfield_eval.argtypes = [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _d]

@@ -307,8 +307,10 @@ int ffa_sc_workspace_layout(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_s
     return ffa_sc_layout_compute(n, ns, nz, nx, out);
 }
 
-int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace) {
-    ffa_ctx_s *c = as_ctx(ctx);
+namespace {
+// Arguments of a solve on a caller workspace (ffa_sc_solve, ffa_sc_solve_field
+// and the push that applies a deferred gather).
+int check_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp, const void *workspace) {
     if (!c || !sp || !workspace) return fail(c, FFA_ERR_INVALID, "ctx, params or workspace is NULL");
     if (!(sp->c > 0.0)) return fail(c, FFA_ERR_INVALID, "speed of light must be positive");
     if (int rc = check_particles(c, p)) return rc;
@@ -322,6 +324,52 @@ int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, voi
                     (long long)p->n);
     if (!sp->seo_r || !sp->seo_pr || !sp->seo_ek_axis || !sp->seo_perimeter || sp->n_ek < 2 || sp->n_fi < 2)
         return fail(c, FFA_ERR_INVALID, "SEO tables missing or smaller than 2x2");
+    return FFA_OK;
+}
+}  // namespace
+
+int ffa_sc_solve_field(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (int rc = check_sc_solve(c, p, sp, workspace)) return rc;
+    ffa_sc_layout L;
+    ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
+    return hip_check(c, launch_sc_solve(c, p, sp, workspace, &L, false), "ffa_sc_solve_field");
+}
+
+int ffa_push_boris_fused_f32_sc(void *ctx, const ffa_particles *p, const ffa_step *steps, const int64_t *inj,
+                                int32_t nsteps, const ffa_sc_params *sp, const void *workspace) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (!c || !steps || !inj) return fail(c, FFA_ERR_INVALID, "ctx, steps or inj is NULL");
+    if (!c->have_machine) return fail(c, FFA_ERR_STATE, "ffa_set_machine has not been called");
+    if (c->host_machine.max_order > 0 && !c->host_machine.taylor_f32)
+        return fail(c, FFA_ERR_INVALID, "the FP32 tier needs a float Taylor map");
+    if (c->mlp_packed != nullptr)
+        return fail(c, FFA_ERR_STATE, "the surrogate push takes its kicks from ffa_sc_solve (surrogate enabled)");
+    if (int rc = check_sc_solve(c, p, sp, workspace)) return rc;
+    if (nsteps < 0 || nsteps > FFA_MAX_FUSED || (size_t)nsteps * 16 > FFA_CTX_INJ_BYTES)
+        return fail(c, FFA_ERR_INVALID, "nsteps %d outside [0, %d]", nsteps, (int)(FFA_CTX_INJ_BYTES / 16));
+    for (int k = 0; k < nsteps; ++k)
+        if (inj[2 * k] < 0 || inj[2 * k + 1] > p->n || inj[2 * k] > inj[2 * k + 1])
+            return fail(c, FFA_ERR_INVALID, "inject range of step %d invalid", k);
+    hipError_t e = hipSuccess;
+    if (nsteps > 0) {
+        e = hipMemcpyAsync(c->dev + FFA_CTX_STEPS_OFF, steps, sizeof(ffa_step) * nsteps, hipMemcpyHostToDevice,
+                           c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->dev + FFA_CTX_INJ_OFF, inj, sizeof(int64_t) * 2 * nsteps, hipMemcpyHostToDevice,
+                               c->stream);
+    }
+    if (e != hipSuccess) return hip_check(c, e, "ffa_push_boris_fused_f32_sc (step table)");
+    ffa_sc_layout L;
+    ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
+    sc_gather_src g;
+    const bool from_phi = sc_gather_inputs(sp, const_cast<void *>(workspace), &L, &g);
+    return hip_check(c, launch_boris_fused_f32_gather(c, p, nsteps, g, from_phi), "ffa_push_boris_fused_f32_sc");
+}
+
+int ffa_sc_solve(void *ctx, const ffa_particles *p, const ffa_sc_params *sp, void *workspace) {
+    ffa_ctx_s *c = as_ctx(ctx);
+    if (int rc = check_sc_solve(c, p, sp, workspace)) return rc;
     ffa_sc_layout L;
     ffa_sc_layout_compute(p->n, sp->ns, sp->nz, sp->nx, &L);
     return hip_check(c, launch_sc_solve(c, p, sp, workspace, &L), "ffa_sc_solve");

@@ -19,6 +19,7 @@
 // roots and one division (the reference recomputes gamma from v each step;
 // the two agree to FP64 rounding).
 #include "ffa_mlp.h"
+#include "ffa_sc_gather.h"
 
 namespace {
 
@@ -220,14 +221,20 @@ constexpr int SUR_BLOCK = 512;
 // Without the surrogate the kernel is held to 96 VGPRs (5 waves per SIMD):
 // the extra wave hides more of the gathers' and constants' latency than the
 // few spilled registers cost.
-template <typename ET, bool SUR>
+//
+// GATHER (1: from the (Ex, Ez) cubes, 2: differentiating phi): the launch
+// first applies the gather and endpoint roll of the SC solve whose results
+// g describes (ffa_sc_solve_field), particle by particle with the solve's
+// own arithmetic, then pushes from the new endpoints; p's endpoint pointers
+// are the ones the solve saw (the caller swaps them after this launch).
+template <typename ET, bool SUR, int GATHER = 0>
 __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK)
 __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
                                                       const ffa_fconst *__restrict__ kc,
                                                       const ffa_step *__restrict__ steps,
                                                       const int64_t *__restrict__ inj, int nsteps,
                                                       const char *__restrict__ sur_w,
-                                                      const mlp::fdesc *__restrict__ md) {
+                                                      const mlp::fdesc *__restrict__ md, sc_gather_src g) {
     const int64_t i = xcd_block() * (int64_t)(SUR ? SUR_BLOCK : BLOCK) + threadIdx.x;
     __shared__ __attribute__((aligned(16))) char lds[SUR ? mlp::FWD_BYTES : 16];
     if constexpr (SUR) {
@@ -243,20 +250,47 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
     const int64_t n = p.n;
     uint8_t fl = inb ? p.flags[ii] : (uint8_t)0;
     double rf = p.rf_phase[ii];
-    const ET *ep = reinterpret_cast<const ET *>(p.esc_pre);
-    const ET *epp = reinterpret_cast<const ET *>(p.esc_prepre);
-    const float e0r = (float)ep[ii], e0z = (float)ep[n + ii], e0f = (float)ep[2 * n + ii];
-    // the endpoint slope, zero unless both endpoints exist (the SC flags do
-    // not change inside a launch): E = E_pre + s (E_pre - E_prepre)
-    const bool sc_both = (fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE);
-    const float d_r = sc_both ? e0r - (float)epp[ii] : 0.0f, d_z = sc_both ? e0z - (float)epp[n + ii] : 0.0f;
-    const float d_f = sc_both ? e0f - (float)epp[2 * n + ii] : 0.0f;
-    const double r0 = p.r[ii], fi0 = p.fi[ii];
+    // the state, loaded before the gather's dependent loads so that their
+    // round trips overlap
+    const double r0 = p.r[ii], fi0 = p.fi[ii], z0 = p.z[ii], vr0 = p.vr[ii], fd0 = p.fidot[ii], vz0 = p.vz[ii];
+    float e0r, e0z, e0f, d_r, d_z, d_f;
+    if constexpr (GATHER != 0) {
+        // the solve's gather and roll for this particle (ffa_sc_gather.h):
+        // the new pre-step field goes into the prepre memory, the pre memory
+        // (on the first solve also overwritten) holds the new prepre field
+        ET *gp = reinterpret_cast<ET *>(p.esc_pre);
+        ET *gpp = reinterpret_cast<ET *>(p.esc_prepre);
+        const ffa_sc_result *res = g.res;
+        const int nx = g.sp.nx, nz = g.sp.nz;
+        const scg::cellw cw = scg::cell_weights(g.pslice[ii], g.px[ii], z0, g.sp.ns, nx, nz, res->xmin, res->zmin,
+                                                1.0 / res->dx, 1.0 / res->dz);
+        const scg::field_scale fsc =
+            GATHER == 2 ? scg::make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0) : scg::field_scale{};
+        float2 a00, a10, a01, a11;
+        scg::corners<GATHER == 2>(cw, g.exz, g.phi, nx, nz, fsc, a00, a10, a01, a11);
+        ET a, b, c;
+        fl = scg::roll<ET>(ii, n, fl, res->bad_grid == 0.0, cw, a00, a10, a01, a11, g.ppr0[ii], g.sp, gp, gpp, a, b, c);
+        e0r = (float)a; e0z = (float)b; e0f = (float)c;
+        const bool sc_both = (fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE);
+        d_r = sc_both ? e0r - (float)gp[ii] : 0.0f;
+        d_z = sc_both ? e0z - (float)gp[n + ii] : 0.0f;
+        d_f = sc_both ? e0f - (float)gp[2 * n + ii] : 0.0f;
+    } else {
+        const ET *ep = reinterpret_cast<const ET *>(p.esc_pre);
+        const ET *epp = reinterpret_cast<const ET *>(p.esc_prepre);
+        e0r = (float)ep[ii]; e0z = (float)ep[n + ii]; e0f = (float)ep[2 * n + ii];
+        // the endpoint slope, zero unless both endpoints exist (the SC flags do
+        // not change inside a launch): E = E_pre + s (E_pre - E_prepre)
+        const bool sc_both = (fl & FFA_F_SCPRE) && (fl & FFA_F_SCPREPRE);
+        d_r = sc_both ? e0r - (float)epp[ii] : 0.0f;
+        d_z = sc_both ? e0z - (float)epp[n + ii] : 0.0f;
+        d_f = sc_both ? e0f - (float)epp[2 * n + ii] : 0.0f;
+    }
     double s0, c0;
     sincos(fi0, &s0, &c0);
-    double x = r0 * c0, y = r0 * s0, z = p.z[ii];
-    const double vr0 = p.vr[ii], rfd = r0 * p.fidot[ii];
-    const double vx0 = vr0 * c0 - rfd * s0, vy0 = vr0 * s0 + rfd * c0, vz0 = p.vz[ii];
+    double x = r0 * c0, y = r0 * s0, z = z0;
+    const double rfd = r0 * fd0;
+    const double vx0 = vr0 * c0 - rfd * s0, vy0 = vr0 * s0 + rfd * c0;
     const double q = m.q, mm = m.m0, cl = m.c;
     const double inv_c2 = 1.0 / (cl * cl);
     // u = gamma v
@@ -414,6 +448,7 @@ hipError_t launch_bfield_eval_f32(ffa_ctx_s *c, const double *r, const double *f
 
 hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps) {
     if (p->n == 0 || nsteps == 0) return hipSuccess;
+    const sc_gather_src gs{};
     const ffa_step *steps = reinterpret_cast<const ffa_step *>(c->dev + FFA_CTX_STEPS_OFF);
     const int64_t *inj = reinterpret_cast<const int64_t *>(c->dev + FFA_CTX_INJ_OFF);
     const mlp::fdesc *md = reinterpret_cast<const mlp::fdesc *>(c->dev + FFA_CTX_MLP_OFF);
@@ -422,18 +457,43 @@ hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t 
         const unsigned g = (unsigned)((p->n + SUR_BLOCK - 1) / SUR_BLOCK);
         if (p->esc_f32)
             hipLaunchKernelGGL((k_boris_fused<float, true>), dim3(g), dim3(SUR_BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
         else
             hipLaunchKernelGGL((k_boris_fused<double, true>), dim3(g), dim3(SUR_BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
         return hipGetLastError();
     }
     const unsigned g = (unsigned)((p->n + BLOCK - 1) / BLOCK);
     if (p->esc_f32)
         hipLaunchKernelGGL((k_boris_fused<float, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
-                           ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+                           ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
     else
         hipLaunchKernelGGL((k_boris_fused<double, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p,
-                           ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md);
+                           ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
+    return hipGetLastError();
+}
+
+hipError_t launch_boris_fused_f32_gather(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps, const sc_gather_src &g,
+                                         bool from_phi) {
+    if (p->n == 0) return hipSuccess;
+    const ffa_step *steps = reinterpret_cast<const ffa_step *>(c->dev + FFA_CTX_STEPS_OFF);
+    const int64_t *inj = reinterpret_cast<const int64_t *>(c->dev + FFA_CTX_INJ_OFF);
+    const mlp::fdesc *md = reinterpret_cast<const mlp::fdesc *>(c->dev + FFA_CTX_MLP_OFF);
+    const unsigned blocks = (unsigned)((p->n + BLOCK - 1) / BLOCK);
+    if (p->esc_f32) {
+        if (from_phi)
+            hipLaunchKernelGGL((k_boris_fused<float, false, 2>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
+        else
+            hipLaunchKernelGGL((k_boris_fused<float, false, 1>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
+    } else {
+        if (from_phi)
+            hipLaunchKernelGGL((k_boris_fused<double, false, 2>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
+        else
+            hipLaunchKernelGGL((k_boris_fused<double, false, 1>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
+    }
     return hipGetLastError();
 }

@@ -157,4 +157,19 @@ hipError_t launch_stage_interp(ffa_ctx_s *c, const float *ex, const float *ez, i
                                double x0, double z0, double dx, double dz, double bl, double *ex_p, double *ez_p,
                                double *ef_p, double *bunch, int64_t n_rows);
 hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp,
-                           void *ws, const ffa_sc_layout *L);
+                           void *ws, const ffa_sc_layout *L, bool gather = true);
+// What the gather of a solve reads (its parameters and workspace regions),
+// for the FP32 push that applies a solve's gather in its prologue.
+struct sc_gather_src {
+    ffa_sc_params sp;
+    const ffa_sc_result *res;
+    const float2 *exz;
+    const float *phi;
+    const double *px, *ppr0;
+    const int32_t *pslice;
+};
+// Fills g from the solve's parameters and workspace; returns true when the
+// gather differentiates phi (no field cubes on this grid).
+bool sc_gather_inputs(const ffa_sc_params *sp, void *ws, const ffa_sc_layout *L, sc_gather_src *g);
+hipError_t launch_boris_fused_f32_gather(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps, const sc_gather_src &g,
+                                         bool from_phi);

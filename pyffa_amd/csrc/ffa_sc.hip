@@ -37,6 +37,7 @@
 // reference's (complex64 FFTs).  Reductions use a fixed block partition and
 // a fixed combine order, so a solve is bit-reproducible run to run.
 #include "ffa_internal.h"
+#include "ffa_sc_gather.h"
 
 namespace {
 
@@ -956,32 +957,9 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv(const float2 *cbuf, floa
 constexpr int FIELD_ROWS = 4;
 constexpr size_t FROM_PHI_MAX_CELLS = (size_t)1 << 23;   // 32 MB of float32 potential
 
-// Scale factors of the stencils: central differences 1 / (2 d gamma^2),
-// one-sided ones 1 / (d gamma^2) (the reference's x-edge and z-edge rows).
-struct field_scale {
-    float cx1, cx2, cz1, cz2;
-};
-
-__device__ __forceinline__ field_scale make_field_scale(double dr, double dz, double gamma) {
-    const double ig2 = 1.0 / (gamma * gamma);
-    return {(float)(ig2 / dr), (float)(0.5 / dr * ig2), (float)(ig2 / dz), (float)(0.5 / dz * ig2)};
-}
-
-// (Ex, Ez) of cell (j, i) from phi at the cell (pc), above and below it in z
-// (pu, pd: rows clamped to the grid) and left and right of it in x (lf, rt:
-// columns clamped to the row).  Shared by k_sc_field and the gather that
-// differentiates phi itself, so both give the same float32 values.  At the
-// x edges the x difference is one-sided and the z difference is the
-// clamped-row one scaled by 1 / (dz gamma^2) (the reference's edge rows);
-// on the z edges (not at an x edge) the z difference is one-sided.
-__device__ __forceinline__ float2 cell_field(float pc, float pu, float pd, float lf, float rt, int i, int j, int nx,
-                                             int nz, const field_scale &f) {
-    const bool xe = i == 0 || i == nx - 1;
-    const bool z0 = !xe && j == 0, z1 = !xe && j == nz - 1;
-    const float dx = i == 0 ? rt - pc : (i == nx - 1 ? pc - lf : rt - lf);
-    const float dz = z0 ? pu - pc : (z1 ? pc - pd : pu - pd);
-    return make_float2(-dx * (xe ? f.cx1 : f.cx2), -dz * ((xe || z0 || z1) ? f.cz1 : f.cz2));
-}
+using scg::cell_field;
+using scg::field_scale;
+using scg::make_field_scale;
 
 // PLANAR: (Ex, Ez) stored as two (ns, nz, nx) cubes ex, ez (the reference's
 // Ex_cube / Ez_cube) instead of the interleaved exz cube.
@@ -1099,30 +1077,6 @@ __global__ __launch_bounds__(BLOCK) void k_fft_rows_inv_field(const float2 *cbuf
     }
 }
 
-// (Ex, Ez) at the gather's four cells (i0 .. i0 + 1, j0 .. j0 + 1, with
-// 0 <= i0 <= nx - 2 and 0 <= j0 <= nz - 2) straight from phi: the 4 x 4
-// stencil block minus its corners, rows and columns clamped to the grid as
-// k_sc_field clamps them.
-__device__ __forceinline__ void cells_field_from_phi(const float *P, int i0, int j0, int nx, int nz,
-                                                     const field_scale &f, float2 &a00, float2 &a10, float2 &a01,
-                                                     float2 &a11) {
-    int64_t row[4];
-    int col[4];
-    for (int k = 0; k < 4; ++k) {
-        const int j = j0 - 1 + k, i = i0 - 1 + k;
-        row[k] = (int64_t)(j < 0 ? 0 : (j > nz - 1 ? nz - 1 : j)) * nx;
-        col[k] = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
-    }
-    const float v01 = P[row[0] + col[1]], v02 = P[row[0] + col[2]];
-    const float v10 = P[row[1] + col[0]], v11 = P[row[1] + col[1]], v12 = P[row[1] + col[2]], v13 = P[row[1] + col[3]];
-    const float v20 = P[row[2] + col[0]], v21 = P[row[2] + col[1]], v22 = P[row[2] + col[2]], v23 = P[row[2] + col[3]];
-    const float v31 = P[row[3] + col[1]], v32 = P[row[3] + col[2]];
-    a00 = cell_field(v11, v21, v01, v10, v12, i0, j0, nx, nz, f);
-    a10 = cell_field(v12, v22, v02, v11, v13, i0 + 1, j0, nx, nz, f);
-    a01 = cell_field(v21, v31, v11, v20, v22, i0, j0 + 1, nx, nz, f);
-    a11 = cell_field(v22, v32, v12, v21, v23, i0 + 1, j0 + 1, nx, nz, f);
-}
-
 // interpolate_fields_to_particles_SBS_numba (src/FFAG_SC_new.py:870-972)
 // fused with the endpoint roll of src/FFAG_track.py:356-382.
 //
@@ -1160,83 +1114,20 @@ __global__ __launch_bounds__(BLOCK) void k_sc_gather_roll(ffa_particles p, ffa_s
         pr[u] = ppr0[j];
     }
     // cells and weights, then every particle's field pairs
-    int64_t cb[GATHER_U];
-    int ci[GATHER_U], cj[GATHER_U];
-    double w00[GATHER_U], w10[GATHER_U], w01[GATHER_U], w11[GATHER_U];
+    scg::cellw cw[GATHER_U];
 #pragma unroll
-    for (int u = 0; u < GATHER_U; ++u) {
-        int s = sl[u];
-        if (s < 0) s = 0; else if (s >= sp.ns) s = sp.ns - 1;
-        const double t = (xv[u] - xmin) * idx, v = (zv[u] - zmin) * idz;
-        const double ct = t > -1e9 ? (t < 1e9 ? t : 1e9) : -1e9, cv = v > -1e9 ? (v < 1e9 ? v : 1e9) : -1e9;
-        int64_t i0 = (int64_t)floor(ct), j0 = (int64_t)floor(cv);
-        if (i0 < 0) i0 = 0; else if (i0 > nx - 2) i0 = nx - 2;
-        if (j0 < 0) j0 = 0; else if (j0 > nz - 2) j0 = nz - 2;
-        const double al = t - (double)i0, be = v - (double)j0;
-        w00[u] = (1.0 - al) * (1.0 - be);
-        w10[u] = al * (1.0 - be);
-        w01[u] = (1.0 - al) * be;
-        w11[u] = al * be;
-        ci[u] = (int)i0;
-        cj[u] = (int)j0;
-        cb[u] = (int64_t)s * nz * nx + j0 * nx + i0;
-    }
+    for (int u = 0; u < GATHER_U; ++u) cw[u] = scg::cell_weights(sl[u], xv[u], zv[u], sp.ns, nx, nz, xmin, zmin, idx, idz);
+    const field_scale fsc = FROM_PHI ? make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0) : field_scale{};
     float2 a00[GATHER_U], a10[GATHER_U], a01[GATHER_U], a11[GATHER_U];
-    if constexpr (FROM_PHI) {
-        const field_scale fsc = make_field_scale(res->dx, res->dz, res->mean_gamma_m1 + 1.0);
 #pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            const int64_t sb = cb[u] - ((int64_t)cj[u] * nx + ci[u]);
-            cells_field_from_phi(phi + sb, ci[u], cj[u], nx, nz, fsc, a00[u], a10[u], a01[u], a11[u]);
-        }
-    } else {
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            a00[u] = exz[cb[u]];
-            a10[u] = exz[cb[u] + 1];
-            a01[u] = exz[cb[u] + nx];
-            a11[u] = exz[cb[u] + nx + 1];
-        }
-    }
+    for (int u = 0; u < GATHER_U; ++u) scg::corners<FROM_PHI>(cw[u], exz, phi, nx, nz, fsc, a00[u], a10[u], a01[u], a11[u]);
 #pragma unroll
     for (int u = 0; u < GATHER_U; ++u) {
         const int64_t i = base + u * BLOCK;
         if (i >= n) break;
-        const bool alive = fl[u] & FFA_F_ALIVE;
-        // the old pre-step field is read only where it survives the solve
-        double o_r = 0.0, o_z = 0.0, o_f = 0.0;
-        if (!(alive && good)) { o_r = (double)ep[i]; o_z = (double)ep[n + i]; o_f = (double)ep[2 * n + i]; }
-        double nr = o_r, nz_ = o_z, nf = o_f;
-        if (alive) {
-            if (good) {
-                const double er = w00[u] * a00[u].x + w10[u] * a10[u].x + w01[u] * a01[u].x + w11[u] * a11[u].x;
-                const double ez = w00[u] * a00[u].y + w10[u] * a10[u].y + w01[u] * a01[u].y + w11[u] * a11[u].y;
-                const double ip = 1.0 / sqrt(1.0 + pr[u] * pr[u]);
-                nr = er * ip;
-                nz_ = ez;
-                nf = er * (-pr[u] * ip);
-            } else if ((fl[u] & FFA_F_SCPRE) && (fl[u] & FFA_F_SCPREPRE)) {
-                // degenerate grid: the reference leaves LocalBunch columns 10-12
-                // as the previous step wrote them (the extrapolated field)
-                nr = o_r + sp.s_last * (o_r - (double)epp[i]);
-                nz_ = o_z + sp.s_last * (o_z - (double)epp[n + i]);
-                nf = o_f + sp.s_last * (o_f - (double)epp[2 * n + i]);
-            }
-        }
-        // The two endpoint buffers trade roles at every solve: the new pre-step
-        // field goes into the old prepre memory (the caller then swaps its
-        // pointers) and the old pre memory, untouched, is the new prepre field.
-        // A live particle writes only its new field; a dead one keeps its old.
-        const ET a = (ET)nr, b = (ET)nz_, c = (ET)nf;
-        epp[i] = a; epp[n + i] = b; epp[2 * n + i] = c;
-        uint8_t nfl;
-        if (sp.first) {
-            ep[i] = a; ep[n + i] = b; ep[2 * n + i] = c;
-            nfl = (uint8_t)((fl[u] & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | (alive ? FFA_F_SCPRE : 0));
-        } else {
-            nfl = (uint8_t)((fl[u] & ~(FFA_F_SCPRE | FFA_F_SCPREPRE)) | ((fl[u] & FFA_F_SCPRE) ? FFA_F_SCPREPRE : 0) |
-                            (alive ? FFA_F_SCPRE : 0));
-        }
+        ET a, b, c;
+        const uint8_t nfl = scg::roll<ET>(i, n, fl[u], good, cw[u], a00[u], a10[u], a01[u], a11[u], pr[u], sp, ep, epp,
+                                          a, b, c);
         if (nfl != fl[u]) p.flags[i] = nfl;
     }
 }
@@ -1335,8 +1226,23 @@ int ffa_sc_layout_compute(int64_t n, int32_t ns, int32_t nz, int32_t nx, ffa_sc_
     return 0;
 }
 
+bool sc_gather_inputs(const ffa_sc_params *sp, void *ws, const ffa_sc_layout *L, sc_gather_src *g) {
+    char *w = reinterpret_cast<char *>(ws);
+    const size_t C = (size_t)sp->ns * sp->nz * sp->nx;
+    g->sp = *sp;
+    g->res = reinterpret_cast<const ffa_sc_result *>(w + L->result);
+    g->exz = reinterpret_cast<const float2 *>(w + L->exz);
+    g->phi = reinterpret_cast<const float *>(w + L->phi);
+    g->px = reinterpret_cast<const double *>(w + L->px);
+    g->ppr0 = reinterpret_cast<const double *>(w + L->ppr0);
+    g->pslice = reinterpret_cast<const int32_t *>(w + L->pslice);
+    return !(sp->field_cubes != 0 || C > FROM_PHI_MAX_CELLS);
+}
+
+// gather = false stops after the field (cubes or phi): the particles'
+// endpoint fields and flags are left for launch_boris_fused_f32_gather.
 hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_params *sp, void *ws,
-                           const ffa_sc_layout *L) {
+                           const ffa_sc_layout *L, bool gather) {
     char *w = reinterpret_cast<char *>(ws);
     int64_t *grid = reinterpret_cast<int64_t *>(w + L->grid);
     int32_t *cell = reinterpret_cast<int32_t *>(w + L->cell_count);
@@ -1403,7 +1309,7 @@ hipError_t launch_sc_solve(ffa_ctx_s *c, const ffa_particles *p, const ffa_sc_pa
             hipLaunchKernelGGL(k_fft_rows_inv_field<false>, g, dim3(BLOCK), lds_b, st, cbuf, phi_out, exz, *sp, res,
                                twt, plx, 0);
     }
-    if (p->n > 0) {
+    if (gather && p->n > 0) {
         const dim3 g((unsigned)((p->n + GATHER_U * BLOCK - 1) / (GATHER_U * BLOCK)));
         if (p->esc_f32) {
             if (cubes)
