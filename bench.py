@@ -11,7 +11,9 @@ One bench step = 1000 Boris push steps: the reference's progress / stop-check
 interval (src/FFAG_track.py:518-577).  It holds 50 SC solves, one stop-check
 reduction and, in the tiers that order particles, one azimuth sort.
 
-The headline value is the FP32-field tier (FP64 state, fused steps).  Beside
+The headline value is the FP32-field tier (FP64 state, fused steps; each
+SC solve stops after the field and the push launch that follows applies its
+gather and endpoint roll).  Beside
 it, on the same workload: the FP64 map-gather path in the reference's
 particle order (the parity path), the FP64 path with the same azimuth
 ordering as the FP32 tier, and that ordered FP64 path with its steps fused
@@ -50,6 +52,7 @@ MLP_FLOP = 2 * (64 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_push_8M.json")
+PUSH_KERNEL = "k_boris_fused<float, false, 1>"     # the FP32 push that applies the gather from the cubes
 VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_fused_push_8M.json")
 
 
@@ -660,20 +663,23 @@ def main():
         out["field_error_fp32_vs_fp64"] = field_error_leg(gp)
         log(f"field error fp32 vs fp64 {out['field_error_fp32_vs_fp64']}")
     # roofline of the dominant kernel of the headline tier: the fused 20-step
-    # push.  Algorithmic HBM bytes per particle per launch: FP64 state read +
-    # write (7 x 8 x 2), flag byte read + write, FP32 SC endpoints Pre/PrePre
-    # (6 x 4).
-    bytes_per_particle = 7 * 8 * 2 + 2 + 6 * 4
+    # push, which also applies the preceding SC solve's gather and endpoint
+    # roll.  Algorithmic HBM bytes per particle per launch: FP64 state read +
+    # write (7 x 8 x 2), flag byte read + write, FP32 SC endpoints (the old
+    # pre-step field read, the new one written: 2 x 3 x 4), the solve's
+    # per-particle gather inputs (slice index 4, x = r - r0 8, pr0 8).  The
+    # field-map and field-cube reads (cache-resident tables shared by many
+    # particles) are not counted.
+    bytes_per_particle = 7 * 8 * 2 + 2 + 2 * 3 * 4 + 4 + 8 + 8
     algo = n * bytes_per_particle
     ach = algo / (k32["push"] * 1e-3) / 1e9
     # the fused push keeps the state in registers for 20 steps, so it is
     # bound by vector-ALU issue rather than HBM: the measured VALU busy
     # fraction (rocprofv3 VALUBusy) is reported beside the HBM fraction
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_boris_fused<float, false>"),
-                       "algorithmic_bytes": algo, "kernel": "k_boris_fused<float> (20 steps)",
-                       "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"],
-                       "valu": measured_valu("k_boris_fused<float, false>")}
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(PUSH_KERNEL),
+                       "algorithmic_bytes": algo, "kernel": "k_boris_fused<float> (20 steps + the solve's gather)",
+                       "kernel_ms": k32["push"], "sc_solve_ms": k32["sc"], "valu": measured_valu(PUSH_KERNEL)}
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(M, tmp, 1, args.surrogate_steps, args.train_iters, args.surrogate_zt)
     if not args.no_mapgen:
