@@ -28,6 +28,9 @@ Further legs (sub-objects of the JSON line):
   with order-4 maps as fallback; beside it the FP32 and FP64 order-4 map
   tiers; training time, field error, matrix-core throughput of the network;
 * ``map_generator`` -- Taylor-map generation on the full map grid;
+* ``spiral_cycle`` (``--cycle-steps K``) -- the 32M-particle spiral-sector
+  acceleration cycle (order-6 maps, spiral gaps, (1024, 512, 512) grid),
+  FP32 tier against the fused, ordered FP64 path, timed after injection;
 * ``cpu_baseline`` -- the OpenMP FP64 oracle on the host cores, on a sample
   of the same workload.
 """
@@ -362,6 +365,52 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
     return out
 
 
+def cycle_leg(tmp, n, steps, precisions=("fp32", "fp64")):
+    """The spiral-sector acceleration cycle (35 degree spiral, order-6 GPU
+    maps, two 200 kV spiral gaps, painted injection of 4 sub-bunches, the
+    (1024, 512, 512) SC grid): the job runs through ``main_track.main`` until
+    the whole bunch is injected (200 ns after injection starts), then
+    ``steps`` more steps are timed, the FP32 tier and the FP64 path with its
+    steps fused and the same azimuth ordering."""
+    import contextlib
+    import torch
+    from pyffa_amd import FFAG_Distribution as FD
+    from pyffa_amd import main_track, workloads as W
+    from pyffa_amd.FFAG_track import FFAG_RungeKutta
+    inj_step = 4000                      # tracking starts 200 ns before injection
+    out = {"particles": n, "sc_grid": [1024, 512, 512], "max_order": 6, "timed_steps": steps}
+    for prec in precisions:
+        folder = os.path.join(tmp, f"cycle_{prec}")
+        os.makedirs(folder, exist_ok=True)
+        path, _ = W.spiral_cycle_job(folder, n_particles=n, grid=(1024, 512, 512, 4.0), max_steps=inj_step + 4200,
+                                     precision=prec, paint_bunches=4, paint_interval_ns=50.0, bunch_length_ns=50.0)
+        FD.set_seed(3)
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(sys.stderr):          # the job's progress lines
+            rk, B, gp = main_track.main(path)
+        torch.cuda.synchronize()
+        log(f"cycle {prec}: job set up and {inj_step + 4200} steps in {time.perf_counter() - t0:.1f} s, "
+            f"injected {B.InjectedNum_Local}")
+        s0, t = rk.last_run["steps"], rk.last_run["t"]
+        timer = LaunchTimer(torch.cuda.current_stream())
+        lp = dict(stop_condition=dict(max_stepsN=s0 + steps, max_turn=1e9, max_time=1.0), time_step=DT,
+                  step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP, precision=prec, verbose=False,
+                  start_step=s0, sort_every=PUSHES, fuse_fp64=prec == "fp64", kernel_timer=timer)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        FFAG_RungeKutta().rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        out[prec] = {"particle_steps_per_s": n * steps / el, "seconds": el,
+                     "alive": int(((B.flags & 2) != 0).sum()), "kernel_ms": timer.result()}
+        log(f"cycle {prec}: {out[prec]}")
+        del rk, B, gp
+    if "fp32" in out and "fp64" in out:
+        out["speedup_fp32_vs_fp64_fused_sorted"] = (out["fp32"]["particle_steps_per_s"] /
+                                                   out["fp64"]["particle_steps_per_s"])
+    return out
+
+
 def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
     """Taylor-map generation on the full map grid of the 12-sector lattice
     (r 8.0-9.3 m at 1 mm: 1301 radii; 30 deg at 0.005 deg: 6000 azimuths)
@@ -567,6 +616,10 @@ def main():
     ap.add_argument("--accuracy-grid", default=None, help="SC grid ns,nx,nz of the accuracy leg")
     ap.add_argument("--accuracy-chunk", type=float, default=50.0, help="turns between moment samples")
     ap.add_argument("--only-accuracy", action="store_true", help="run the accuracy leg alone")
+    ap.add_argument("--cycle-steps", type=int, default=0,
+                    help="timed steps of the 32M spiral acceleration-cycle leg (0: leg off)")
+    ap.add_argument("--cycle-particles", type=int, default=32_000_000)
+    ap.add_argument("--only-cycle", action="store_true", help="run the spiral-cycle leg alone")
     ap.add_argument("--no-mapgen", action="store_true", help="skip the Taylor-map generator leg")
     ap.add_argument("--only-mapgen", action="store_true", help="run the Taylor-map generator leg alone")
     ap.add_argument("--tune-turns", type=int, default=0,
@@ -603,6 +656,10 @@ def main():
                                                  runs=tuple(args.tune_runs.split(",")), progress=args.tune_progress,
                                                  grid=_grid_arg(args.tune_grid), save_dir=args.tune_save,
                                                  load_dir=args.tune_load)}), flush=True)
+        return
+    if args.only_cycle:
+        print(json.dumps({"spiral_cycle": cycle_leg(tmp, args.cycle_particles, args.cycle_steps or 200)}),
+              flush=True)
         return
     if args.only_accuracy:
         acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","),
@@ -687,6 +744,8 @@ def main():
     if args.tune_turns > 0:
         out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
                                      progress=args.tune_progress, grid=_grid_arg(args.tune_grid))
+    if args.cycle_steps > 0:
+        out["spiral_cycle"] = cycle_leg(tmp, args.cycle_particles, args.cycle_steps)
     if args.accuracy_turns > 0:
         out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
                                             args.accuracy_tiers.split(","), chunk_turns=args.accuracy_chunk,
