@@ -77,6 +77,37 @@ __device__ __forceinline__ void radius_f32(float xf, float yf, float &r2, float 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) f4v gfloat4;
 
+// Orders 1..NO of the Taylor gather written out for one map order (the
+// kernel instance for order-NO maps; order 2 is written out in fields_f32).
+// Each of the cell's two runs (6 NO floats: corner (i, .) then corner
+// (i + 1, .), orders 1..NO, (Bz, Bphi, Br) each) is read as one block, which
+// the compiler issues as 16-byte loads instead of 4 NO 12-byte loads; the
+// four corners' sums per order are completed before the order's z power
+// multiplies them, as in the other paths.
+template <int NO>
+__device__ __forceinline__ void taylor_runs(gfloat *a0, gfloat *a1, float w00, float w01, float w10, float w11,
+                                            float z2, float z, float &Bz, float &Bfi, float &Br) {
+    constexpr int L = 6 * NO, H = 3 * NO;
+    float c0[L], c1[L];
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+        c0[q] = a0[q];
+        c1[q] = a1[q];
+    }
+    float ze = z2, zo = z2 * z;
+#pragma unroll
+    for (int n = 0; n < NO; ++n) {
+        const float bz = w00 * c0[3 * n] + w01 * c1[3 * n] + w10 * c0[H + 3 * n] + w11 * c1[H + 3 * n];
+        const float bf = w00 * c0[3 * n + 1] + w01 * c1[3 * n + 1] + w10 * c0[H + 3 * n + 1] + w11 * c1[H + 3 * n + 1];
+        const float br = w00 * c0[3 * n + 2] + w01 * c1[3 * n + 2] + w10 * c0[H + 3 * n + 2] + w11 * c1[H + 3 * n + 2];
+        Bz += bz * ze;
+        Bfi += bf * zo;
+        Br += br * zo;
+        ze *= z2;
+        zo *= z2;
+    }
+}
+
 // Field at (r, th, z): th is the azimuth within a turn of the branch cut
 // (harmonics, gap direction, spiral-gap window), fim the same azimuth reduced
 // to the sector [0, P) in FP64 before rounding (the map index and the
@@ -87,6 +118,10 @@ typedef const __attribute__((address_space(1))) f4v gfloat4;
 // intervals per sector), indexed in FP64 from locd when there is no spiral,
 // so that the steep edges see the FP64 azimuth rather than its FP32
 // rounding.
+//
+// ORD > 0: the kernel instance for order-ORD maps (its own register budget),
+// with the gather written out for that order.
+template <int ORD = 0>
 __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, float r, float inv_r, float th, float fim,
                                            double locd, float z, float xf, float yf, float v0, double rf, float &Bz,
                                            float &Br, float &Bfi, float &Er, float &Efi, float &Ez, float &sin_rf,
@@ -134,7 +169,10 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
         gfloat *T = (gfloat *)(k->T);          // global address space: global_load, not flat
         const unsigned cell = __umul24((unsigned)j, (unsigned)k->NX) + (unsigned)i;
         float ze = z2, zo = z2 * z;
-        if (k->no == 2) {
+        if constexpr (ORD > 0) {
+            gfloat *p0 = T + cell * (unsigned)(3 * ORD), *p1 = p0 + (unsigned)(k->NX * 3 * ORD);
+            taylor_runs<ORD>(p0, p1, w00, w01, w10, w11, z2, z, Bz, Bfi, Br);
+        } else if (k->no == 2) {
             gfloat *a0 = T + cell * 6u, *a1 = a0 + 6u * (unsigned)k->NX;
             // 12 floats per run: order 1 (Bz, Bfi, Br), order 2, for i and i + 1
             float c0[12], c1[12];
@@ -217,6 +255,9 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
 // lane runs the step loop to the end so the wave-wide matrix-core work sees
 // all 64 lanes.
 constexpr int SUR_BLOCK = 512;
+#ifndef ORD_WAVES
+#define ORD_WAVES 4
+#endif
 
 // Without the surrogate the kernel is held to 96 VGPRs (5 waves per SIMD):
 // the extra wave hides more of the gathers' and constants' latency than the
@@ -227,9 +268,9 @@ constexpr int SUR_BLOCK = 512;
 // g describes (ffa_sc_solve_field), particle by particle with the solve's
 // own arithmetic, then pushes from the new endpoints; p's endpoint pointers
 // are the ones the solve saw (the caller swaps them after this launch).
-template <typename ET, bool SUR, int GATHER = 0>
+template <typename ET, bool SUR, int GATHER = 0, int ORD = 0>
 __global__ __launch_bounds__(SUR ? SUR_BLOCK : BLOCK)
-__attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
+__attribute__((amdgpu_waves_per_eu(SUR ? 1 : (ORD > 0 ? ORD_WAVES : 5), SUR ? 8 : (ORD > 0 ? ORD_WAVES : 5)))) void k_boris_fused(ffa_particles p, const ffa_machine *__restrict__ mp,
                                                       const ffa_fconst *__restrict__ kc,
                                                       const ffa_step *__restrict__ steps,
                                                       const int64_t *__restrict__ inj, int nsteps,
@@ -338,7 +379,7 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
         const float sc = (float)s.sc_s;
         const float esr = fmaf(sc, d_r, e0r), esz = fmaf(sc, d_z, e0z), esf = fmaf(sc, d_f, e0f);
         float Bz, Br, Bfi, Er, Efi, Ez, srf;
-        fields_f32(kc, rr, ir, th, fim, locd, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
+        fields_f32<ORD>(kc, rr, ir, th, fim, locd, zf, xf, yf, (float)s.v0, rf, Bz, Br, Bfi, Er, Efi, Ez, srf,
                    SUR ? (!use_sur && md->fallback != 0) : true);
         if constexpr (SUR) {
             Bz += sBz;
@@ -413,6 +454,7 @@ __attribute__((amdgpu_waves_per_eu(SUR ? 1 : 5, SUR ? 8 : 5))) void k_boris_fuse
 // evaluates it at a position, widened to double: the FP64 Cartesian
 // position rounded to FP32 for the radius, the FP64 azimuth reduced to the
 // sector before rounding, fields_f32 on the FP32 map.
+template <int ORD>
 __global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__restrict__ kc, int nsectors,
                                                            const double *__restrict__ r, const double *__restrict__ fi,
                                                            const double *__restrict__ z, int64_t n,
@@ -430,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__r
     const double locd = thd - Pd * floor(thd * ((double)nsectors / FFA_TWO_PI));
     const float th = (float)thd, fim = (float)locd;
     float Bz, Br, Bfi, Er, Efi, Ez, srf;
-    fields_f32(kc, rr, ir, th, fim, locd, zf, xf, yf, 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
+    fields_f32<ORD>(kc, rr, ir, th, fim, locd, zf, xf, yf, 0.0f, 0.0, Bz, Br, Bfi, Er, Efi, Ez, srf);
     out[i] = (double)Bz;
     out[n + i] = (double)Bfi;
     out[2 * n + i] = (double)Br;
@@ -441,9 +483,41 @@ __global__ __launch_bounds__(BLOCK) void k_bfield_eval_f32(const ffa_fconst *__r
 hipError_t launch_bfield_eval_f32(ffa_ctx_s *c, const double *r, const double *fi, const double *z, int64_t n,
                                   double *out) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bfield_eval_f32, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
-                       ctx_fconst_dev(c), c->host_machine.nsectors, r, fi, z, n, out);
+    // the push's instance: written-out order-6 gather for order-6 maps
+    if (c->host_machine.max_order == 6)
+        hipLaunchKernelGGL(k_bfield_eval_f32<6>, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                           c->stream, ctx_fconst_dev(c), c->host_machine.nsectors, r, fi, z, n, out);
+    else
+        hipLaunchKernelGGL(k_bfield_eval_f32<0>, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                           c->stream, ctx_fconst_dev(c), c->host_machine.nsectors, r, fi, z, n, out);
     return hipGetLastError();
+}
+
+// The push without the surrogate: the endpoint type of the particles, the
+// gather mode GATHER, and the kernel instance for order-6 maps when the
+// machine has them (its own register budget; every other order runs the
+// general instance).
+template <int GATHER>
+static void launch_push(ffa_ctx_s *c, const ffa_particles *p, unsigned blocks, const ffa_step *steps,
+                        const int64_t *inj, int32_t nsteps, const mlp::fdesc *md, const sc_gather_src &g) {
+    const bool ord6 = c->host_machine.max_order == 6;
+    const ffa_machine *m = ctx_machine_dev(c);
+    const ffa_fconst *k = ctx_fconst_dev(c);
+    if (p->esc_f32) {
+        if (ord6)
+            hipLaunchKernelGGL((k_boris_fused<float, false, GATHER, 6>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               m, k, steps, inj, nsteps, nullptr, md, g);
+        else
+            hipLaunchKernelGGL((k_boris_fused<float, false, GATHER, 0>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               m, k, steps, inj, nsteps, nullptr, md, g);
+    } else {
+        if (ord6)
+            hipLaunchKernelGGL((k_boris_fused<double, false, GATHER, 6>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               m, k, steps, inj, nsteps, nullptr, md, g);
+        else
+            hipLaunchKernelGGL((k_boris_fused<double, false, GATHER, 0>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
+                               m, k, steps, inj, nsteps, nullptr, md, g);
+    }
 }
 
 hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t nsteps) {
@@ -464,12 +538,7 @@ hipError_t launch_boris_fused_f32(ffa_ctx_s *c, const ffa_particles *p, int32_t 
         return hipGetLastError();
     }
     const unsigned g = (unsigned)((p->n + BLOCK - 1) / BLOCK);
-    if (p->esc_f32)
-        hipLaunchKernelGGL((k_boris_fused<float, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p, ctx_machine_dev(c),
-                           ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
-    else
-        hipLaunchKernelGGL((k_boris_fused<double, false>), dim3(g), dim3(BLOCK), 0, c->stream, *p,
-                           ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, w, md, gs);
+    launch_push<0>(c, p, g, steps, inj, nsteps, md, gs);
     return hipGetLastError();
 }
 
@@ -480,20 +549,7 @@ hipError_t launch_boris_fused_f32_gather(ffa_ctx_s *c, const ffa_particles *p, i
     const int64_t *inj = reinterpret_cast<const int64_t *>(c->dev + FFA_CTX_INJ_OFF);
     const mlp::fdesc *md = reinterpret_cast<const mlp::fdesc *>(c->dev + FFA_CTX_MLP_OFF);
     const unsigned blocks = (unsigned)((p->n + BLOCK - 1) / BLOCK);
-    if (p->esc_f32) {
-        if (from_phi)
-            hipLaunchKernelGGL((k_boris_fused<float, false, 2>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
-        else
-            hipLaunchKernelGGL((k_boris_fused<float, false, 1>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
-    } else {
-        if (from_phi)
-            hipLaunchKernelGGL((k_boris_fused<double, false, 2>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
-        else
-            hipLaunchKernelGGL((k_boris_fused<double, false, 1>), dim3(blocks), dim3(BLOCK), 0, c->stream, *p,
-                               ctx_machine_dev(c), ctx_fconst_dev(c), steps, inj, nsteps, nullptr, md, g);
-    }
+    if (from_phi) launch_push<2>(c, p, blocks, steps, inj, nsteps, md, g);
+    else launch_push<1>(c, p, blocks, steps, inj, nsteps, md, g);
     return hipGetLastError();
 }

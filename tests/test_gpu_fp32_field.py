@@ -1,14 +1,15 @@
 """The FP32 tier's static field (its own evaluation: position rounded to
 FP32, the push's atan2, FP32 order-0 field and FP32 Taylor map; ffa_bfield_eval
 mode 3) against the FP64 path's field (mode 0) over the map: the tier's
-field error, stated against the 1e-4 relative bound."""
+field error, stated against the 1e-4 relative bound.  Order 6 runs the push's
+kernel instance for order-6 maps (the gather written out for that order)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("max_order", [2, 4])
+@pytest.mark.parametrize("max_order", [2, 4, 6])
 def test_fp32_field_error_below_1e4(torch_cuda, max_order):
     from pyffa_amd import workloads as W
     from pyffa_amd.FFAG_Field import field_at_points

@@ -119,3 +119,48 @@ def test_spiral_cycle_matches_the_oracle_and_accelerates(tmp_path):
     gain = np.median(ct.ConvertVrzek2Przek_boris(M)[both, 5] - ek0[both])
     print(f"median energy gain over {turns} turns: {gain:.4f} MeV ({both.sum()} particles)", flush=True)
     assert abs(gain - 0.2 * turns) < 0.25 * 0.2 * turns
+
+
+def test_spiral_cycle_fp32_tier_close_to_fp64(tmp_path):
+    """The FP32 tier on the spiral cycle (order-6 maps: the push's kernel
+    instance for that order; spiral gaps, SC, painting) against the FP64
+    path from the same painted bunch: bunch centroid and rms emittances of
+    the particles alive in both agree as on the order-2 lattice."""
+    from pyffa_amd import FFAG_Distribution as FD
+    from pyffa_amd import main_track, workloads as W
+    n = 4000
+    stop = INJ_STEP + 1200
+    print("[case] spiral cycle fp32 tier vs fp64, 4000 particles", flush=True)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        folder = tmp_path / prec
+        folder.mkdir()
+        path, _ = W.spiral_cycle_job(str(folder), n_particles=n, grid=(32, 32, 32, 4.0), max_steps=stop,
+                                     precision=prec, paint_bunches=4, paint_interval_ns=50.0, bunch_length_ns=50.0)
+        FD.set_seed(7)
+        rk, B, gp = main_track.main(path)
+        assert gp.Bmap.max_order == 6
+        out[prec] = B.LocalBunch
+    a, b = out["fp32"], out["fp64"]
+    injected = int((b[:, 7] != 0).sum())
+    print(f"[case] injected {injected}, alive fp32 {int((a[:, 8] != 0).sum())} fp64 {int((b[:, 8] != 0).sum())}",
+          flush=True)
+    assert 0 < injected <= n
+    assert np.mean(a[:, 8] != b[:, 8]) < 0.005
+    both = (a[:, 8] != 0) & (b[:, 8] != 0)
+    assert both.sum() > 0.5 * injected
+    ma, mb = _moments(a[both]), _moments(b[both])
+    print(ma, mb, flush=True)
+    assert abs(ma["r"] - mb["r"]) < 1e-5 and abs(ma["z"] - mb["z"]) < 1e-5
+    assert abs(ma["er"] / mb["er"] - 1) < 1e-3 and abs(ma["ez"] / mb["ez"] - 1) < 1e-3
+
+
+def _moments(M):
+    """Centroid and rms emittances (r, vr/v) and (z, vz/v)."""
+    v = np.sqrt(M[:, 1] ** 2 + M[:, 3] ** 2 + (M[:, 0] * M[:, 5]) ** 2)
+
+    def emit(x, xp):
+        x, xp = x - x.mean(), xp - xp.mean()
+        return float(np.sqrt(np.mean(x * x) * np.mean(xp * xp) - np.mean(x * xp) ** 2))
+    return dict(r=float(M[:, 0].mean()), z=float(M[:, 2].mean()), er=emit(M[:, 0], M[:, 1] / v),
+                ez=emit(M[:, 2], M[:, 3] / v))
