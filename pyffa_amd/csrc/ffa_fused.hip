@@ -255,6 +255,9 @@ __device__ __forceinline__ void fields_f32(const ffa_fconst *__restrict__ k, flo
 // lane runs the step loop to the end so the wave-wide matrix-core work sees
 // all 64 lanes.
 constexpr int SUR_BLOCK = 512;
+// Waves per SIMD of the order-6 instance: its written-out gather holds two
+// 144-byte runs; at 4 waves (128 registers) a few values spill outside the
+// step loop, at 3 (no spill) it ran 8 % slower on the 32M spiral cycle.
 #ifndef ORD_WAVES
 #define ORD_WAVES 4
 #endif
