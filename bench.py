@@ -58,7 +58,7 @@ MLP_FLOP = 2 * (64 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_push_8M.json")
-PUSH_KERNEL = "k_boris_fused<float, false, 1>"     # the FP32 push that applies the gather from the cubes
+PUSH_KERNEL = "k_boris_fused<float, false, 1, 0>"  # the FP32 push that applies the gather from the cubes
 VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_fused_push_8M.json")
 
 
