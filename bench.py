@@ -307,16 +307,18 @@ def _grid_arg(text):
     return GRID if not text else tuple(int(v) for v in text.split(",")) + (GRID[3],)
 
 
-def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
+def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID, sort_every=SORT_EVERY):
     """Tracks the same bunch for ``turns`` turns (SC, RF, aperture) in the
     requested runs: "fp32" (FP32 tier), "fp64" (FP64 path) and "fp64p" (FP64
     path from r and z moved by one ulp: the problem's own sensitivity).  The
     closed-orbit moments are recorded every ``chunk_turns`` turns; the
     relative emittance differences of fp32 and fp64p against fp64 are
     reported side by side.  Every run reorders its particles by azimuth
-    every 1000 steps (for the FP64 path this changes only the partition of
+    every ``sort_every`` steps (for the FP64 path this changes only the partition of
     the SC moment sums, a rounding-level difference like the one fp64p
-    measures)."""
+    measures; ``sort_every`` 1000 reproduces the runs recorded before the
+    FP32 tier's default became 200).  The FP64 runs fuse each block of steps
+    into one launch (bit-identical to one launch per step)."""
     import numpy as np
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
     from pyffa_amd.FFAG_track import FFAG_RungeKutta
@@ -327,7 +329,7 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
     steps_per_turn = seo.T0[3] / DT
     total = int(round(turns * steps_per_turn / SC_STEP))
     n_chunks = max(1, int(round(turns / chunk_turns)))
-    out = {"particles": n, "turns": turns, "sc_grid": list(grid[:3]), "steps": total * SC_STEP,
+    out = {"particles": n, "turns": turns, "sc_grid": list(grid[:3]), "steps": total * SC_STEP, "sort_every": sort_every,
            "moments": "about the closed orbit r0(Ek, phi), pr0(Ek, phi) of the SEO tables"}
     for run in tiers:
         prec = "fp32" if run == "fp32" else "fp64"
@@ -344,7 +346,7 @@ def accuracy_leg(tmp, n, turns, tiers, chunk_turns=50, grid=GRID):
             last = total * (c + 1) // n_chunks
             lp = dict(stop_condition=dict(max_stepsN=last * SC_STEP, max_turn=1e9, max_time=1.0), time_step=DT,
                       step_dumps=None, enable_SC=True, SC_type=1, SC_step=SC_STEP, precision=prec, verbose=False,
-                      start_step=done * SC_STEP, sort_every=PUSHES)
+                      start_step=done * SC_STEP, sort_every=sort_every, fuse_fp64=prec == "fp64")
             rk.rk4_solve_dt_bunch3_withSC_boris(None, t, B, gp, lp)
             t, done = rk.last_run["t"], last
             S = {k: B.ordered(k) for k in ("r", "vr", "z", "vz", "fi", "fidot", "flags")}
@@ -618,6 +620,7 @@ def main():
     ap.add_argument("--accuracy-tiers", default="fp32,fp64,fp64p")
     ap.add_argument("--accuracy-grid", default=None, help="SC grid ns,nx,nz of the accuracy leg")
     ap.add_argument("--accuracy-chunk", type=float, default=50.0, help="turns between moment samples")
+    ap.add_argument("--accuracy-sort", type=int, default=SORT_EVERY, help="steps between azimuth sorts, every run")
     ap.add_argument("--only-accuracy", action="store_true", help="run the accuracy leg alone")
     ap.add_argument("--cycle-steps", type=int, default=0,
                     help="timed steps of the 32M spiral acceleration-cycle leg (0: leg off)")
@@ -666,7 +669,8 @@ def main():
         return
     if args.only_accuracy:
         acc = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns, args.accuracy_tiers.split(","),
-                           chunk_turns=args.accuracy_chunk, grid=_grid_arg(args.accuracy_grid))
+                           chunk_turns=args.accuracy_chunk, grid=_grid_arg(args.accuracy_grid),
+                           sort_every=args.accuracy_sort)
         print(json.dumps({"accuracy_long": acc}), flush=True)
         return
     from pyffa_amd import workloads as W
@@ -753,7 +757,7 @@ def main():
     if args.accuracy_turns > 0:
         out["accuracy_long"] = accuracy_leg(tmp, args.accuracy_particles, args.accuracy_turns,
                                             args.accuracy_tiers.split(","), chunk_turns=args.accuracy_chunk,
-                                            grid=_grid_arg(args.accuracy_grid))
+                                            grid=_grid_arg(args.accuracy_grid), sort_every=args.accuracy_sort)
     if not args.no_cpu:
         cb = cpu_baseline(M, gp, min(args.cpu_particles, n), args.cpu_blocks, n)
         log(f"cpu baseline {cb}")
