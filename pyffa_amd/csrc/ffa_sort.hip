@@ -184,21 +184,36 @@ __global__ __launch_bounds__(SBLOCK) void k_gather_particles(ffa_particles s, ff
     if (k >= s.n) return;
     const int64_t j = k < count ? (int64_t)order[k] : k;
     const int64_t n = s.n;
-    d.r[k] = s.r[j];
-    d.vr[k] = s.vr[j];
-    d.z[k] = s.z[j];
-    d.vz[k] = s.vz[j];
-    d.fi[k] = s.fi[j];
-    d.fidot[k] = s.fidot[j];
-    d.rf_phase[k] = s.rf_phase[j];
-    d.flags[k] = s.flags[j];
+    // every field is loaded before any is stored (the compiler cannot tell
+    // the source arrays from the destination ones, so interleaved loads and
+    // stores would wait one by one); measured the same at 8M, where the
+    // scattered reads, not the per-thread chain, bound the kernel
+    const double r = s.r[j], vr = s.vr[j], z = s.z[j], vz = s.vz[j], fi = s.fi[j], fd = s.fidot[j];
+    const double rf = s.rf_phase[j];
+    const uint8_t fl = s.flags[j];
     const ET *ep = reinterpret_cast<const ET *>(s.esc_pre), *epp = reinterpret_cast<const ET *>(s.esc_prepre);
-    ET *dp = reinterpret_cast<ET *>(d.esc_pre), *dpp = reinterpret_cast<ET *>(d.esc_prepre);
+    ET e[3], f[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        dp[c * n + k] = ep[c * n + j];
-        dpp[c * n + k] = epp[c * n + j];
+        e[c] = ep[c * n + j];
+        f[c] = epp[c * n + j];
     }
-    pid_d[k] = pid_s[j];
+    const int64_t id = pid_s[j];
+    d.r[k] = r;
+    d.vr[k] = vr;
+    d.z[k] = z;
+    d.vz[k] = vz;
+    d.fi[k] = fi;
+    d.fidot[k] = fd;
+    d.rf_phase[k] = rf;
+    d.flags[k] = fl;
+    ET *dp = reinterpret_cast<ET *>(d.esc_pre), *dpp = reinterpret_cast<ET *>(d.esc_prepre);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dp[c * n + k] = e[c];
+        dpp[c * n + k] = f[c];
+    }
+    pid_d[k] = id;
 }
 
 }  // namespace
