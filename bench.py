@@ -474,7 +474,7 @@ def mapgen_leg(orders=(2, 6), reps=5, cpu_rows=2, cpu_order=4, cpu=True):
 
 
 def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), progress=False, grid=GRID,
-             save_dir=None, load_dir=None):
+             save_dir=None, load_dir=None, sort_every=SORT_EVERY):
     """Incoherent tunes with space charge: the bunch of ``accuracy_leg`` is
     tracked ``turns`` + 1.5 turns in each run ("bare": FP64 without SC;
     "fp64", "fp32": with SC; "fp64p": FP64 with SC from r and z moved by one
@@ -486,7 +486,10 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
     ``save_dir``: each run's per-particle tunes are written there as
     ``tune_<run>.json``; ``load_dir``: runs saved there by an earlier call
     (same workload) join the comparison, so the runs can be split over
-    several calls."""
+    several calls.  ``sort_every``: steps between azimuth sorts in every run
+    (1000 reproduces the runs recorded before the FP32 tier's default became
+    200); the FP64 runs fuse each block of steps into one launch
+    (bit-identical to one launch per step)."""
     import numpy as np
     from pyffa_amd import diagnostics as D
     from pyffa_amd.FFAG_Bunch import FFAG_Bunch
@@ -530,7 +533,7 @@ def tune_leg(tmp, n, turns, n_track, runs=("bare", "fp64", "fp32", "fp64p"), pro
         log(f"tune {run}: tracking {n} particles for {total} steps")
         lp = dict(stop_condition=dict(max_stepsN=total, max_turn=1e9, max_time=1.0), time_step=DT,
                   step_dumps=dumps, enable_SC=run != "bare", SC_type=1, SC_step=SC_STEP, precision=prec,
-                  verbose=progress, sort_every=PUSHES)
+                  verbose=progress, sort_every=sort_every, fuse_fp64=prec == "fp64")
         rk.rk4_solve_dt_bunch3_withSC_boris(None, 0.0, B, gp, lp)
         pd.save_all_particles()
         ids, qx, qz = D.particle_tunes(D.load_position_dump(folder, 0.0), seo, turns=turns)
@@ -634,6 +637,7 @@ def main():
     ap.add_argument("--tune-tracked", type=int, default=2000)
     ap.add_argument("--tune-grid", default=None, help="SC grid ns,nx,nz of the tune leg")
     ap.add_argument("--only-tune", action="store_true", help="run the tune leg alone")
+    ap.add_argument("--tune-sort", type=int, default=SORT_EVERY, help="steps between azimuth sorts, every run")
     ap.add_argument("--tune-runs", default="bare,fp64,fp32,fp64p", help="runs of the tune leg in this call")
     ap.add_argument("--tune-save", default=None, help="directory for each run's per-particle tunes")
     ap.add_argument("--tune-load", default=None, help="directory of runs saved by an earlier call")
@@ -661,7 +665,7 @@ def main():
         print(json.dumps({"tune_shift": tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
                                                  runs=tuple(args.tune_runs.split(",")), progress=args.tune_progress,
                                                  grid=_grid_arg(args.tune_grid), save_dir=args.tune_save,
-                                                 load_dir=args.tune_load)}), flush=True)
+                                                 load_dir=args.tune_load, sort_every=args.tune_sort)}), flush=True)
         return
     if args.only_cycle:
         print(json.dumps({"spiral_cycle": cycle_leg(tmp, args.cycle_particles, args.cycle_steps or 200)}),
@@ -751,7 +755,8 @@ def main():
         out["map_generator"] = mapgen_leg(cpu=not args.no_cpu)
     if args.tune_turns > 0:
         out["tune_shift"] = tune_leg(tmp, args.tune_particles, args.tune_turns, args.tune_tracked,
-                                     progress=args.tune_progress, grid=_grid_arg(args.tune_grid))
+                                     progress=args.tune_progress, grid=_grid_arg(args.tune_grid),
+                                     sort_every=args.tune_sort)
     if args.cycle_steps > 0:
         out["spiral_cycle"] = cycle_leg(tmp, args.cycle_particles, args.cycle_steps)
     if args.accuracy_turns > 0:
