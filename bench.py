@@ -10,8 +10,8 @@ ramp, the aperture r 8.0-9.3 m |z| < 5 cm, Boris pusher, dt = 0.05 ns.
 One bench step = 1000 Boris push steps: the reference's progress / stop-check
 interval (src/FFAG_track.py:518-577).  It holds 50 SC solves, one stop-check
 reduction and, in the tiers that order particles, five azimuth sorts (every
-200 steps, the FP32 tier's default; the accuracy and tune legs sort every
-1000 steps, as when their recorded results were made).
+200 steps, the FP32 tier's default; the accuracy and tune legs sort at the
+same interval unless --accuracy-sort / --tune-sort says otherwise).
 
 The headline value is the FP32-field tier (FP64 state, fused steps; each
 SC solve stops after the field and the push launch that follows applies its
